@@ -1,0 +1,130 @@
+/*
+ * sadcs_hip.h - C ABI of the MI355X (gfx950) suffix array and BWT construction.
+ *
+ * Plain C: pointers, sizes and status codes, no HIP or C++ types. Each entry point replaces one
+ * call of the reference's GPU interface, src/cuda/gpusa.h (cited per function), with the same
+ * meaning of arguments, the same refusal rules and the same status order:
+ *   SADCS_OK, SADCS_UNAVAILABLE, SADCS_NOT_ENOUGH_MEMORY, SADCS_FAILED  (gpusa.h:22-27).
+ * The C++ API of gpusa.h itself is provided on top of this by gpusa.cpp.
+ *
+ * Output conventions (divsufsort's, as the reference): the suffix array holds n int32 positions;
+ * the BWT is the BWT of text plus a sentinel without the sentinel, n bytes, with the primary
+ * index (the sentinel's row) returned separately (gpusa.h:104-108).
+ */
+#ifndef SADCS_HIP_H
+#define SADCS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SADCS_OK = 0,
+    SADCS_UNAVAILABLE = 1,         /* no device or no usable driver */
+    SADCS_NOT_ENOUGH_MEMORY = 2,   /* the input does not fit in free device memory or the context */
+    SADCS_FAILED = 3               /* a HIP call or an internal check failed; see sadcs_hip_last_error() */
+};
+
+/* gpusa.h:29-35 (GpuDevice). */
+typedef struct sadcs_device {
+    char name[256];
+    int32_t compute_major;
+    int32_t compute_minor;
+    int64_t total_bytes;
+    int64_t free_bytes;
+} sadcs_device;
+
+/* gpusa.h:37-53 (GpuStats); zeroed at the start of every call. Seconds are host wall clock
+ * around device-synchronised phases, as in the reference; device_seconds is text resident on the
+ * device to outputs complete on the device. */
+typedef struct sadcs_stats {
+    int64_t device_bytes;        /* peak device memory of the call, including caller-owned device buffers */
+    int64_t ranking_bytes;
+    int64_t class_sort_bytes;
+    int64_t merge_bytes;
+    int32_t pinned;              /* transfers went through pinned staging or pinned caller buffers */
+    int32_t character_passes;    /* sample ranking: character passes */
+    int32_t doubling_rounds;     /* sample ranking: prefix-doubling rounds */
+    int32_t cover;               /* the difference cover modulus used */
+    double setup_seconds;
+    double upload_seconds;
+    double device_seconds;
+    double ranking_seconds;
+    double class_sort_seconds;
+    double merge_seconds;
+    double bwt_seconds;
+    double download_seconds;
+    double merge_kernel_seconds; /* HIP-event time of the top-level tile-merge kernel on the context's stream */
+    int64_t merge_kernel_elements; /* elements that kernel merged (the whole text) */
+} sadcs_stats;
+
+/* Construction parameters; zero fields take the defaults. */
+typedef struct sadcs_params {
+    int32_t cover;               /* 32 (default) or 128 */
+    uint32_t splitter_spacing;   /* G: every G-th element of a list is a splitter */
+    uint32_t tile_splitters;     /* K: sorted splitters per merge tile */
+} sadcs_params;
+
+typedef struct sadcs_ctx sadcs_ctx;
+
+/* gpusa.h:76 (gpuQueryDevice): the first device. */
+int sadcs_hip_query(sadcs_device* device);
+
+/* gpusa.h:58-60 (gpuSampleSize): number of sample positions of a text of n bytes for the cover. */
+int64_t sadcs_hip_sample_size(int32_t n, int32_t cover);
+
+/* gpusa.h:79 (gpuRequiredBytes): peak device bytes of a suffix array + BWT build of n bytes,
+ * -1 if unknown. */
+int64_t sadcs_hip_required_bytes(int32_t n);
+
+/* gpusa.h:84-95 (gpuCreateContext / gpuDestroyContext): device memory reserved for inputs up
+ * to capacity bytes, kept across calls. Refuses with SADCS_NOT_ENOUGH_MEMORY, allocating nothing,
+ * if the planned peak plus 256 MiB exceeds the free device memory. Not thread-safe: one context per
+ * thread. params may be null. Destroy ignores null. */
+int sadcs_hip_ctx_create(int32_t capacity, const sadcs_params* params, sadcs_ctx** context);
+void sadcs_hip_ctx_destroy(sadcs_ctx* context);
+
+/* gpusa.h:97-108 (gpuSuffixArray and gpuBwt): builds from host memory. sa (n int32) and/or
+ * bwt (n bytes, may alias text) select the outputs; both non-null builds both in one merge.
+ * primary receives the BWT's primary index (-1 until success; 0 for n = 0). A null context
+ * uses a temporary one. n over the context's capacity gives SADCS_NOT_ENOUGH_MEMORY. n <= 0 is Ok. */
+int sadcs_hip_build(sadcs_ctx* context, const uint8_t* text, int32_t n, int32_t* sa, uint8_t* bwt,
+                    int32_t* primary, sadcs_stats* stats);
+
+/* The same with the text already on the device and the outputs left there (no host transfers).
+ * d_text: n bytes followed by at least 64 zero bytes, 8-byte aligned. d_sa (4n bytes) and d_bwt
+ * (n bytes) are device buffers or null. No counterpart in the reference: the device-resident
+ * form of gpuSuffixArray/gpuBwt for callers that keep texts in HBM. */
+int sadcs_hip_build_device(sadcs_ctx* context, const void* d_text, int32_t n, void* d_sa, void* d_bwt,
+                           int32_t* primary, sadcs_stats* stats);
+
+/* gpusa.h:110-114 (gpuRankSample): uploads text and ranks the sample only;
+ * ranks receives sadcs_hip_sample_size(n, 32) entries: ranks[pack(p)] for each sample position p. */
+int sadcs_hip_rank_sample(const uint8_t* text, int32_t n, uint32_t* ranks, sadcs_stats* stats);
+
+/* gpusa.h:116-120 (gpuAllocatePinned / gpuFreePinned). */
+void* sadcs_hip_alloc_pinned(size_t bytes);
+void sadcs_hip_free_pinned(void* pointer);
+
+/* gpusa.h:122-123 (gpuLastError): "" or a description of the last failure of this thread. */
+const char* sadcs_hip_last_error(void);
+
+/* ---- batching: many texts in one construction (no counterpart in the reference) ----
+ * texts: the concatenated texts; lengths[i] bytes each (>= 1). Outputs are concatenated in the
+ * same order: sa receives lengths[i] int32 per text (local positions), bwt lengths[i] bytes per
+ * text, primaries one int32 per text. Either output may be null. The batch runs as one pipeline:
+ * every sort and merge is segmented by a text-index key. */
+int sadcs_hip_batch_build(sadcs_ctx* context, const uint8_t* texts, const int64_t* lengths, int32_t count,
+                          int32_t* sa, uint8_t* bwt, int32_t* primaries, sadcs_stats* stats);
+
+/* Device bytes a batch of these lengths needs. */
+int64_t sadcs_hip_batch_required_bytes(const int64_t* lengths, int32_t count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SADCS_HIP_H */

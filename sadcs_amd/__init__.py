@@ -1,0 +1,273 @@
+"""Python binding of the MI355X suffix array / BWT construction (the C ABI of include/sadcs_hip.h).
+
+The names mirror the reference's GPU interface, src/cuda/gpusa.h: gpu_suffix_array, gpu_bwt,
+gpu_rank_sample, GpuContext, GpuStatus, GpuStats, gpu_required_bytes, gpu_last_error. All work
+runs in the HIP library built by sadcs_amd/build.py; importing this module without it raises.
+"""
+import ctypes
+import enum
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "libsadcs_hip.so")
+
+
+class GpuStatus(enum.IntEnum):
+    """gpusa.h:22-27."""
+    Ok = 0
+    Unavailable = 1
+    NotEnoughMemory = 2
+    Failed = 3
+
+
+class GpuDevice(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 256), ("compute_major", ctypes.c_int32), ("compute_minor", ctypes.c_int32),
+                ("total_bytes", ctypes.c_int64), ("free_bytes", ctypes.c_int64)]
+
+
+class GpuStats(ctypes.Structure):
+    """gpusa.h:37-53, plus the cover used."""
+    _fields_ = [("device_bytes", ctypes.c_int64), ("ranking_bytes", ctypes.c_int64),
+                ("class_sort_bytes", ctypes.c_int64), ("merge_bytes", ctypes.c_int64),
+                ("pinned", ctypes.c_int32), ("character_passes", ctypes.c_int32),
+                ("doubling_rounds", ctypes.c_int32), ("cover", ctypes.c_int32),
+                ("setup_seconds", ctypes.c_double), ("upload_seconds", ctypes.c_double),
+                ("device_seconds", ctypes.c_double), ("ranking_seconds", ctypes.c_double),
+                ("class_sort_seconds", ctypes.c_double), ("merge_seconds", ctypes.c_double),
+                ("bwt_seconds", ctypes.c_double), ("download_seconds", ctypes.c_double),
+                ("merge_kernel_seconds", ctypes.c_double), ("merge_kernel_elements", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class GpuParams(ctypes.Structure):
+    _fields_ = [("cover", ctypes.c_int32), ("splitter_spacing", ctypes.c_uint32), ("tile_splitters", ctypes.c_uint32)]
+
+
+class GpuError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{GpuStatus(status).name}: {message}")
+        self.status = GpuStatus(status)
+
+
+def _load():
+    if not os.path.exists(LIBRARY_PATH):
+        raise ImportError(f"{LIBRARY_PATH} is missing: build it with `python -m sadcs_amd.build`")
+    lib = ctypes.CDLL(LIBRARY_PATH)
+    P, i32, i64, u8p = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
+    lib.sadcs_hip_query.argtypes = [ctypes.POINTER(GpuDevice)]
+    lib.sadcs_hip_query.restype = ctypes.c_int
+    lib.sadcs_hip_sample_size.argtypes = [i32, i32]
+    lib.sadcs_hip_sample_size.restype = i64
+    lib.sadcs_hip_required_bytes.argtypes = [i32]
+    lib.sadcs_hip_required_bytes.restype = i64
+    lib.sadcs_hip_ctx_create.argtypes = [i32, ctypes.POINTER(GpuParams), ctypes.POINTER(P)]
+    lib.sadcs_hip_ctx_create.restype = ctypes.c_int
+    lib.sadcs_hip_ctx_destroy.argtypes = [P]
+    lib.sadcs_hip_ctx_destroy.restype = None
+    lib.sadcs_hip_build.argtypes = [P, u8p, i32, P, P, ctypes.POINTER(i32), ctypes.POINTER(GpuStats)]
+    lib.sadcs_hip_build.restype = ctypes.c_int
+    lib.sadcs_hip_build_device.argtypes = [P, P, i32, P, P, ctypes.POINTER(i32), ctypes.POINTER(GpuStats)]
+    lib.sadcs_hip_build_device.restype = ctypes.c_int
+    lib.sadcs_hip_rank_sample.argtypes = [u8p, i32, P, ctypes.POINTER(GpuStats)]
+    lib.sadcs_hip_rank_sample.restype = ctypes.c_int
+    lib.sadcs_hip_alloc_pinned.argtypes = [ctypes.c_size_t]
+    lib.sadcs_hip_alloc_pinned.restype = P
+    lib.sadcs_hip_free_pinned.argtypes = [P]
+    lib.sadcs_hip_free_pinned.restype = None
+    lib.sadcs_hip_last_error.argtypes = []
+    lib.sadcs_hip_last_error.restype = ctypes.c_char_p
+    lib.sadcs_hip_batch_build.argtypes = [P, u8p, ctypes.POINTER(i64), i32, P, P, P, ctypes.POINTER(GpuStats)]
+    lib.sadcs_hip_batch_build.restype = ctypes.c_int
+    lib.sadcs_hip_batch_required_bytes.argtypes = [ctypes.POINTER(i64), i32]
+    lib.sadcs_hip_batch_required_bytes.restype = i64
+    return lib
+
+
+lib = _load()
+
+# The symbols include/sadcs_hip.h declares (checked by the CPU tests).
+EXPORTED_SYMBOLS = (
+    "sadcs_hip_query", "sadcs_hip_sample_size", "sadcs_hip_required_bytes", "sadcs_hip_ctx_create",
+    "sadcs_hip_ctx_destroy", "sadcs_hip_build", "sadcs_hip_build_device", "sadcs_hip_rank_sample",
+    "sadcs_hip_alloc_pinned", "sadcs_hip_free_pinned", "sadcs_hip_last_error", "sadcs_hip_batch_build",
+    "sadcs_hip_batch_required_bytes",
+)
+
+
+def gpu_last_error():
+    """gpusa.h:122-123."""
+    return lib.sadcs_hip_last_error().decode(errors="replace")
+
+
+def _check(status):
+    if status != GpuStatus.Ok:
+        raise GpuError(status, gpu_last_error())
+
+
+def gpu_query_device():
+    """gpusa.h:76: (status, GpuDevice)."""
+    device = GpuDevice()
+    status = lib.sadcs_hip_query(ctypes.byref(device))
+    return GpuStatus(status), device
+
+
+def gpu_sample_size(n, cover=32):
+    """gpusa.h:58-60."""
+    return lib.sadcs_hip_sample_size(n, cover)
+
+
+def gpu_required_bytes(n):
+    """gpusa.h:79."""
+    return lib.sadcs_hip_required_bytes(n)
+
+
+def _as_bytes(text):
+    if isinstance(text, np.ndarray):
+        if text.dtype != np.uint8:
+            raise TypeError("text must be uint8")
+        return np.ascontiguousarray(text)
+    return np.frombuffer(bytes(text), dtype=np.uint8)
+
+
+class GpuContext:
+    """gpusa.h:84-95: device memory for inputs up to capacity bytes, kept across calls."""
+
+    def __init__(self, capacity, cover=32, splitter_spacing=0, tile_splitters=0):
+        params = GpuParams(cover, splitter_spacing, tile_splitters)
+        handle = ctypes.c_void_p()
+        _check(lib.sadcs_hip_ctx_create(int(capacity), ctypes.byref(params), ctypes.byref(handle)))
+        self.handle = handle
+        self.capacity = capacity
+        self.cover = cover
+
+    def close(self):
+        if self.handle:
+            lib.sadcs_hip_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _handle(context):
+    return context.handle if context is not None else None
+
+
+def gpu_build(text, suffix_array=True, bwt=True, context=None, stats=None):
+    """Suffix array and/or BWT of text in one construction.
+
+    Returns (sa or None, bwt bytes or None, primary). Raises GpuError on failure.
+    """
+    t = _as_bytes(text)
+    n = t.size
+    sa_out = np.empty(n, dtype=np.int32) if suffix_array else None
+    bwt_out = np.empty(n, dtype=np.uint8) if bwt else None
+    primary = ctypes.c_int32(-1)
+    st = stats if stats is not None else GpuStats()
+    status = lib.sadcs_hip_build(_handle(context), t.ctypes.data if n else None, n,
+                                 sa_out.ctypes.data if sa_out is not None and n else None,
+                                 bwt_out.ctypes.data if bwt_out is not None and n else None,
+                                 ctypes.byref(primary), ctypes.byref(st))
+    _check(status)
+    return sa_out, (bwt_out.tobytes() if bwt_out is not None else None), primary.value
+
+
+def gpu_suffix_array(text, context=None, stats=None):
+    """gpusa.h:101-102: the suffix array (int32 numpy array)."""
+    return gpu_build(text, True, False, context, stats)[0]
+
+
+def gpu_bwt(text, context=None, stats=None):
+    """gpusa.h:107-108: (bwt bytes, primary index) in buildBwt()'s convention."""
+    _, b, primary = gpu_build(text, False, True, context, stats)
+    return b, primary
+
+
+def gpu_build_status(text, n=None, context=None, suffix_array=True, bwt=False):
+    """The raw status of a build (for refusal tests); n may exceed len(text) only for n <= 0."""
+    t = _as_bytes(text)
+    n = t.size if n is None else n
+    sa_out = np.empty(max(n, 1), dtype=np.int32) if suffix_array else None
+    bwt_out = np.empty(max(n, 1), dtype=np.uint8) if bwt else None
+    primary = ctypes.c_int32(-1)
+    st = GpuStats()
+    status = lib.sadcs_hip_build(_handle(context), t.ctypes.data if t.size else None, n,
+                                 sa_out.ctypes.data if sa_out is not None else None,
+                                 bwt_out.ctypes.data if bwt_out is not None else None,
+                                 ctypes.byref(primary), ctypes.byref(st))
+    return GpuStatus(status)
+
+
+def gpu_rank_sample(text, stats=None):
+    """gpusa.h:110-114: ranks[pack(p)] of the cover-32 sample positions p."""
+    t = _as_bytes(text)
+    n = t.size
+    ranks = np.empty(max(gpu_sample_size(n), 1), dtype=np.uint32)
+    st = stats if stats is not None else GpuStats()
+    _check(lib.sadcs_hip_rank_sample(t.ctypes.data if n else None, n, ranks.ctypes.data, ctypes.byref(st)))
+    return ranks[:gpu_sample_size(n)]
+
+
+def gpu_build_device(context, d_text, n, d_sa=None, d_bwt=None, stats=None):
+    """Device-resident build. d_text: device pointer to n bytes plus 64 zero bytes (8-byte aligned);
+    d_sa (4n bytes) and d_bwt (n bytes): device pointers or None. Returns the primary index."""
+    primary = ctypes.c_int32(-1)
+    st = stats if stats is not None else GpuStats()
+    _check(lib.sadcs_hip_build_device(_handle(context), ctypes.c_void_p(d_text), int(n),
+                                      ctypes.c_void_p(d_sa) if d_sa else None,
+                                      ctypes.c_void_p(d_bwt) if d_bwt else None,
+                                      ctypes.byref(primary), ctypes.byref(st)))
+    return primary.value
+
+
+def gpu_batch_build(texts, suffix_array=True, bwt=True, context=None, stats=None):
+    """Many texts in one construction. Returns a list of (sa, bwt bytes, primary) per text."""
+    arrays = [_as_bytes(t) for t in texts]
+    lengths = np.array([a.size for a in arrays], dtype=np.int64)
+    if (lengths < 1).any():
+        raise ValueError("batched texts need at least one byte each")
+    joined = np.concatenate(arrays) if arrays else np.empty(0, dtype=np.uint8)
+    total = int(lengths.sum())
+    sa_out = np.empty(max(total, 1), dtype=np.int32) if suffix_array else None
+    bwt_out = np.empty(max(total, 1), dtype=np.uint8) if bwt else None
+    primaries = np.empty(max(len(arrays), 1), dtype=np.int32)
+    st = stats if stats is not None else GpuStats()
+    _check(lib.sadcs_hip_batch_build(_handle(context), joined.ctypes.data, lengths.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                     len(arrays), sa_out.ctypes.data if sa_out is not None else None,
+                                     bwt_out.ctypes.data if bwt_out is not None else None,
+                                     primaries.ctypes.data, ctypes.byref(st)))
+    results = []
+    offset = 0
+    for i, n in enumerate(lengths):
+        n = int(n)
+        results.append((sa_out[offset:offset + n].copy() if sa_out is not None else None,
+                        bwt_out[offset:offset + n].tobytes() if bwt_out is not None else None,
+                        int(primaries[i]) if bwt_out is not None else None))
+        offset += n
+    return results
+
+
+def gpu_allocate_pinned(size):
+    """gpusa.h:116-120: (pointer, numpy view) of page-locked host memory, or (None, None)."""
+    pointer = lib.sadcs_hip_alloc_pinned(size)
+    if not pointer:
+        return None, None
+    view = np.ctypeslib.as_array(ctypes.cast(pointer, ctypes.POINTER(ctypes.c_uint8)), shape=(max(size, 1),))
+    return pointer, view
+
+
+def gpu_free_pinned(pointer):
+    lib.sadcs_hip_free_pinned(pointer)

@@ -1,0 +1,103 @@
+/*
+ * classes.h - Phase (b): the sorted list of every class of non-sample suffixes.
+ *
+ * Class d holds the positions whose next sample position is d characters ahead. For i in class
+ * d >= 1, i + 1 is in class d - 1, so suffix i is ordered by (T[i], suffix i + 1) and class d is
+ * class d - 1 walked in order, restricted to positions whose predecessor is in class d, stably
+ * sorted by that predecessor's character (src/cuda/classsort.cuh:1-20, :184-244). The walk starts
+ * with the text's end n, standing for the empty suffix: position n - 1 then precedes every other
+ * suffix starting with T[n - 1] (src/cuda/classsort.cuh:14-17, :77-80). Class 0 is the sample in
+ * rank order, written by phase (a).
+ *
+ * Each element carries four characters T[q], T[q-1], T[q-2], T[q-3] from the top byte down; the
+ * next class shifts them left a byte, so only every fourth class reads the text
+ * (src/cuda/classsort.cuh:8-12).
+ *
+ * Here the walk, the filter and the key are a Source functor of the radix sort, so one class is
+ * one filtered stable pass: histogram, scan, scatter. In a batch the text index sits above the
+ * characters in a 64-bit key and adds passes on its bits; the ends of all texts lead the walk,
+ * which keeps each end ahead of its own text's elements with the same character.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "dev_common.h"
+#include "prim.h"
+
+namespace sadcs {
+
+template <int V>
+__device__ __forceinline__ uint32_t classOf(uint32_t p);
+
+// T[q], T[q-1], T[q-2], T[q-3] from the top byte down; bytes before `start` read as 0.
+__device__ __forceinline__ uint32_t characterKey(const uint8_t* __restrict__ bytes, uint32_t q, uint32_t start) {
+    uint32_t key = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+        key = (key << 8) | (q >= start + i ? bytes[q - i] : 0u);
+    }
+    return key;
+}
+
+// The walk over class d - 1 with every text's end in front, filtered to the elements whose
+// predecessor is in class d. Key: text index above the four characters (batch) or the characters.
+template <int V, class Texts, class K>
+struct ClassWalkSource {
+    const uint8_t* bytes;
+    const uint32_t* previous;      // class d - 1, sorted
+    const K* previousKeys;         // its four-character keys (low 32 bits), or null to read the text
+    uint32_t previousCount;
+    const Segment* segmentTable;
+    uint32_t segments;
+    uint32_t n;                    // positions in the text buffer (a batch: all texts and gaps)
+    Texts texts;
+    int d;
+
+    __device__ __forceinline__ bool operator()(uint32_t k, K& key, uint32_t& value) const {
+        uint32_t w, start, segment;
+        bool fromEnd = k < segments;
+        if (fromEnd) {
+            if constexpr (Texts::kBatch) {
+                const Segment s = segmentTable[k];
+                w = s.end;
+                start = s.start;
+            } else {
+                w = n;
+                start = 0;
+            }
+            segment = k;
+        } else {
+            w = previous[k - segments];
+            if (w >= n) {
+                return false;  // not a position: drop it (the class then comes out short and fails its check)
+            }
+            if constexpr (Texts::kBatch) {
+                segment = texts.segmentOf(w);
+                start = segmentTable[segment].start;
+            } else {
+                segment = 0;
+                start = 0;
+            }
+        }
+        if (w <= start || classOf<V>(w - 1) != static_cast<uint32_t>(d)) {
+            return false;
+        }
+        const uint32_t q = w - 1;
+        const uint32_t chars = (previousKeys && !fromEnd)
+            ? static_cast<uint32_t>(previousKeys[k - segments]) << 8
+            : characterKey(bytes, q, start);
+        if constexpr (sizeof(K) == 8) {
+            key = (static_cast<K>(segment) << 32) | chars;
+        } else {
+            key = chars;
+        }
+        value = q;
+        return true;
+    }
+};
+
+// Class 0 of a batch needs no work: phase (a) ranks by (text, suffix), and the sorted sample
+// array is indexed by those global ranks.
+
+}  // namespace sadcs

@@ -1,0 +1,1750 @@
+/*
+ * engine.hip - The construction on one gfx950 device, behind the C ABI of include/sadcs_hip.h.
+ *
+ * One translation unit: the phases (rank.h, classes.h, merge.h) share the __constant__ cover
+ * tables defined here. The host side plans memory by running the allocation sequence of a build
+ * against a dry-run arena (so the plan is exactly what the build allocates, fragmentation
+ * included), reserves that much once per context, and sequences the phases on one stream.
+ *
+ * Reference counterparts: src/cuda/gpusa.cu (host driver: planMemory 171-201, checkMemory 203-218,
+ * uploadAndRank 226-267, GpuContext 321-396, buildOnDevice 406-466, gpuBwt 544-605,
+ * gpuRankSample 632-676).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "../../include/sadcs_hip.h"
+#include "classes.h"
+#include "cover.h"
+#include "dev_common.h"
+#include "merge.h"
+#include "prim.h"
+#include "rank.h"
+
+namespace sadcs {
+
+// ==== device cover tables =======================================================================
+
+__constant__ CoverTables<32> gCover32;
+__constant__ CoverTables<128> gCover128;
+
+template <>
+__device__ __forceinline__ const CoverTables<32>& deviceCover<32>() {
+    return gCover32;
+}
+
+template <>
+__device__ __forceinline__ const CoverTables<128>& deviceCover<128>() {
+    return gCover128;
+}
+
+template <int V>
+__device__ __forceinline__ uint32_t packPosition(uint32_t p) {
+    return (p >> DifferenceCover<V>::kLog2) * CoverSet<V>::kSize + deviceCover<V>().below[p & (V - 1)];
+}
+
+template <int V>
+__device__ __forceinline__ uint32_t unpackIndex(uint32_t t) {
+    return (t / CoverSet<V>::kSize) * V + deviceCover<V>().residue[t % CoverSet<V>::kSize];
+}
+
+template <int V>
+__device__ __forceinline__ uint32_t classOf(uint32_t p) {
+    return deviceCover<V>().distance[p & (V - 1)];
+}
+
+// ==== small kernels ==============================================================================
+
+__global__ void fillWordsKernel(uint32_t* __restrict__ out, uint64_t count, uint32_t value) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x) {
+        out[i] = value;
+    }
+}
+
+__global__ void fillBytesKernel(uint8_t* __restrict__ out, uint64_t count, uint8_t value) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x) {
+        out[i] = value;
+    }
+}
+
+// Byte histogram of the text positions that belong to a text.
+template <class Texts>
+__global__ void byteHistogramKernel(const uint8_t* __restrict__ bytes, uint32_t span, Texts texts, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t local[256];
+    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) {
+        local[x] = 0;
+    }
+    __syncthreads();
+    for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < span; p += uint64_t(gridDim.x) * blockDim.x) {
+        if (!Texts::kBatch || p < texts.endOf(static_cast<uint32_t>(p))) {
+            atomicAdd(&local[bytes[p]], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) {
+        if (local[x]) {
+            atomicAdd(&counts[x], local[x]);
+        }
+    }
+}
+
+// Rows of a merge level that is a single tile: zeros, then the lengths and their sum.
+__global__ void singleTileRowsKernel(ListSet lists, uint32_t* __restrict__ rows) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const int lc = lists.count;
+        uint32_t total = 0;
+        for (int l = 0; l < lc; l++) {
+            rows[l] = 0;
+            rows[lc + 1 + l] = lists.length[l];
+            total += lists.length[l];
+        }
+        rows[lc] = 0;
+        rows[2 * lc + 1] = total;
+    }
+}
+
+// BWT byte 0 of every text: T[end - 1], the row of the sentinel.
+__global__ void firstBwtBytesKernel(const uint8_t* __restrict__ bytes, const Segment* __restrict__ segments, uint32_t count, uint8_t* __restrict__ bwt) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < count) {
+        bwt[segments[s].rowBase] = bytes[segments[s].end - 1];
+    }
+}
+
+// ==== errors =====================================================================================
+
+thread_local char tLastError[512] = "";
+
+void setError(const char* format, ...) __attribute__((format(printf, 1, 2)));
+
+void setError(const char* format, ...) {
+    va_list args;
+    va_start(args, format);
+    vsnprintf(tLastError, sizeof tLastError, format, args);
+    va_end(args);
+}
+
+int statusOf(hipError_t e) {
+    return e == hipErrorOutOfMemory ? SADCS_NOT_ENOUGH_MEMORY : SADCS_FAILED;
+}
+
+#define SADCS_HIP(call)                                                                   \
+    do {                                                                                  \
+        const hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                           \
+            setError("%s: %s", #call, hipGetErrorString(e_));                             \
+            return statusOf(e_);                                                          \
+        }                                                                                 \
+    } while (0)
+
+#define SADCS_TRY(call)                 \
+    do {                                \
+        const int s_ = (call);          \
+        if (s_ != SADCS_OK) {           \
+            return s_;                  \
+        }                               \
+    } while (0)
+
+double secondsSince(std::chrono::steady_clock::time_point start) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+}
+
+int bitWidth(uint64_t x) {
+    int b = 0;
+    while (x) {
+        b++;
+        x >>= 1;
+    }
+    return b;
+}
+
+// ==== arena ======================================================================================
+
+// First-fit allocator over one device block. In dry-run mode it has no memory and only records
+// the high-water mark of the same allocation sequence, which becomes the plan.
+class Arena {
+ public:
+    static constexpr size_t kAlign = 256;
+
+    void reset(uint8_t* base, size_t capacity, bool dry) {
+        m_base = base;
+        m_capacity = capacity;
+        m_dry = dry;
+        m_free.assign(1, Block{0, capacity});
+        m_used.clear();
+        m_highWater = 0;
+        m_live = 0;
+    }
+
+    bool dry() const {
+        return m_dry;
+    }
+
+    size_t highWater() const {
+        return m_highWater;
+    }
+
+    void resetHighWater() {
+        size_t top = 0;
+        for (const Block& b : m_used) {
+            top = std::max(top, b.offset + b.size);
+        }
+        m_highWater = top;
+    }
+
+    template <class T>
+    int allocate(T*& pointer, size_t count) {
+        const size_t bytes = std::max<size_t>(kAlign, (count * sizeof(T) + kAlign - 1) / kAlign * kAlign);
+        for (size_t i = 0; i < m_free.size(); i++) {
+            if (m_free[i].size >= bytes) {
+                const size_t offset = m_free[i].offset;
+                m_free[i].offset += bytes;
+                m_free[i].size -= bytes;
+                if (m_free[i].size == 0) {
+                    m_free.erase(m_free.begin() + static_cast<std::ptrdiff_t>(i));
+                }
+                m_used.push_back(Block{offset, bytes});
+                m_highWater = std::max(m_highWater, offset + bytes);
+                m_live += bytes;
+                pointer = reinterpret_cast<T*>((m_dry ? reinterpret_cast<uint8_t*>(kAlign) : m_base) + offset);
+                return SADCS_OK;
+            }
+        }
+        setError("device arena exhausted: %zu bytes requested, capacity %zu", bytes, m_capacity);
+        return SADCS_NOT_ENOUGH_MEMORY;
+    }
+
+    template <class T>
+    void release(T*& pointer) {
+        if (!pointer) {
+            return;
+        }
+        const size_t offset = static_cast<size_t>(reinterpret_cast<uint8_t*>(pointer) -
+                                                  (m_dry ? reinterpret_cast<uint8_t*>(kAlign) : m_base));
+        for (size_t i = 0; i < m_used.size(); i++) {
+            if (m_used[i].offset == offset) {
+                insertFree(m_used[i]);
+                m_live -= m_used[i].size;
+                m_used.erase(m_used.begin() + static_cast<std::ptrdiff_t>(i));
+                break;
+            }
+        }
+        pointer = nullptr;
+    }
+
+ private:
+    struct Block {
+        size_t offset;
+        size_t size;
+    };
+
+    void insertFree(Block b) {
+        size_t i = 0;
+        while (i < m_free.size() && m_free[i].offset < b.offset) {
+            i++;
+        }
+        m_free.insert(m_free.begin() + static_cast<std::ptrdiff_t>(i), b);
+        if (i + 1 < m_free.size() && m_free[i].offset + m_free[i].size == m_free[i + 1].offset) {
+            m_free[i].size += m_free[i + 1].size;
+            m_free.erase(m_free.begin() + static_cast<std::ptrdiff_t>(i + 1));
+        }
+        if (i > 0 && m_free[i - 1].offset + m_free[i - 1].size == m_free[i].offset) {
+            m_free[i - 1].size += m_free[i].size;
+            m_free.erase(m_free.begin() + static_cast<std::ptrdiff_t>(i));
+        }
+    }
+
+    uint8_t* m_base = nullptr;
+    size_t m_capacity = 0;
+    bool m_dry = true;
+    std::vector<Block> m_free;
+    std::vector<Block> m_used;
+    size_t m_highWater = 0;
+    size_t m_live = 0;
+};
+
+// ==== jobs =======================================================================================
+
+struct Params {
+    int cover = 32;
+    uint32_t g = 64;        // splitter spacing
+    uint32_t k = 16;        // sorted splitters per tile
+
+    // Zero fields keep the defaults of the cover: tiles of at most K*G + C*(G-1) records must fit
+    // the LDS (cover 32: 7 classes, 48-byte records; cover 128: 31 classes, 72-byte records).
+    void apply(const sadcs_params* p) {
+        if (p && p->cover) {
+            cover = p->cover;
+        }
+        g = cover == 128 ? 32 : 64;
+        k = 16;
+        if (p && p->splitter_spacing) {
+            g = p->splitter_spacing;
+        }
+        if (p && p->tile_splitters) {
+            k = p->tile_splitters;
+        }
+    }
+};
+
+// One construction: texts on the device (segments of one position space) and what to produce.
+struct Job {
+    const uint8_t* text = nullptr;          // device, span + 64 bytes, zero padded
+    uint32_t span = 0;                      // positions, gaps between texts included
+    std::vector<Segment> segments;          // host copy
+    const Segment* deviceSegments = nullptr;
+    const uint32_t* blockSegment = nullptr; // batch only
+    bool batch = false;
+    bool rankOnly = false;
+    uint32_t* sa = nullptr;                 // device outputs, null = not wanted / allocate
+    uint8_t* bwt = nullptr;
+    bool wantSA = false;
+    bool wantBWT = false;
+    uint32_t* firstRows = nullptr;          // device, one per text (BWT)
+    uint32_t maxLength = 0;
+    uint64_t totalLength = 0;               // positions in texts (output rows)
+};
+
+struct PhaseTimes {
+    double ranking = 0, classes = 0, merge = 0, bwt = 0;
+    double mergeKernel = 0;
+    uint64_t mergeElements = 0;
+    int characterPasses = 0, doublingRounds = 0;
+    size_t rankingBytes = 0, classBytes = 0, mergeBytes = 0;
+};
+
+}  // namespace sadcs
+
+// ==== the context ================================================================================
+
+struct sadcs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int32_t capacity = 0;
+    sadcs::Params params;
+    uint8_t* arenaBase = nullptr;
+    size_t arenaBytes = 0;
+    sadcs::Arena arena;
+    uint32_t* hostWords = nullptr;     // pinned, for readbacks
+    uint8_t* staging[2] = {};          // pinned staging chunks
+    hipEvent_t stagingDone[2] = {};
+    hipEvent_t kernelTimer[2] = {};    // around the top-level merge kernel
+    int sms = 0;
+};
+
+namespace sadcs {
+
+constexpr size_t kStagingChunk = size_t(8) << 20;
+constexpr int64_t kMemoryReserve = int64_t(256) << 20;
+constexpr size_t kTextPadding = 64;
+
+bool gTablesReady[2] = {false, false};
+
+int ensureTables() {
+    if (!gTablesReady[0]) {
+        static CoverTables<32> t32;
+        t32.fill();
+        SADCS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(gCover32), &t32, sizeof t32));
+        gTablesReady[0] = true;
+    }
+    if (!gTablesReady[1]) {
+        static CoverTables<128> t128;
+        t128.fill();
+        SADCS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(gCover128), &t128, sizeof t128));
+        gTablesReady[1] = true;
+    }
+    return SADCS_OK;
+}
+
+template <int V>
+uint64_t classSizeOf(uint64_t n, int d) {
+    uint64_t size = 0;
+    for (int r = 0; r < V; r++) {
+        if (DifferenceCover<V>::nextSampleDistance(r) == d) {
+            size += n / V + (static_cast<uint64_t>(r) < n % V ? 1 : 0);
+        }
+    }
+    return size;
+}
+
+template <int V>
+uint32_t mergeCapacity(uint32_t g, uint32_t k) {
+    return k * g + DifferenceCover<V>::kClasses * (g - 1);
+}
+
+// ==== the pipeline ===============================================================================
+
+template <int V, class Texts>
+class Pipeline {
+    using Cover = DifferenceCover<V>;
+    static constexpr int kClasses = Cover::kClasses;
+    static constexpr bool kBatch = Texts::kBatch;
+
+ public:
+    Pipeline(sadcs_ctx& ctx, Job& job, Texts texts) : m_ctx(ctx), m_arena(ctx.arena), m_job(job), m_texts(texts) {
+        m_stream = ctx.stream;
+        m_dry = m_arena.dry();
+        m_segments = static_cast<uint32_t>(job.segments.size());
+        m_samples = 0;
+        for (const Segment& s : job.segments) {
+            m_samples += static_cast<uint32_t>(Cover::pack(s.end - s.start));
+        }
+        m_ctxMerge.text = reinterpret_cast<const uint64_t*>(job.text);
+        m_ctxMerge.segments = job.deviceSegments;
+        m_ctxMerge.n = job.span;
+    }
+
+    ~Pipeline() {
+        releaseAll();
+    }
+
+    PhaseTimes times;
+
+    int run() {
+        std::chrono::steady_clock::time_point start = std::chrono::steady_clock::now();
+        SADCS_TRY(rankSample());
+        SADCS_TRY(sync());
+        times.ranking = secondsSince(start);
+        times.rankingBytes = m_arena.highWater();
+        if (m_job.rankOnly) {
+            return SADCS_OK;  // the caller reads ranks() before the pipeline goes
+        }
+        start = std::chrono::steady_clock::now();
+        SADCS_TRY(sortClasses());
+        SADCS_TRY(sync());
+        times.classes = secondsSince(start);
+        times.classBytes = m_arena.highWater();
+        start = std::chrono::steady_clock::now();
+        SADCS_TRY(merge());
+        SADCS_TRY(sync());
+        times.merge = secondsSince(start);
+        times.mergeBytes = m_arena.highWater();
+        return SADCS_OK;
+    }
+
+    const uint32_t* ranks() const {
+        return m_ranks;
+    }
+
+    uint32_t samples() const {
+        return m_samples;
+    }
+
+ private:
+    sadcs_ctx& m_ctx;
+    Arena& m_arena;
+    Job& m_job;
+    Texts m_texts;
+    hipStream_t m_stream;
+    bool m_dry;
+    uint32_t m_segments;
+    uint32_t m_samples;
+    MergeCtx m_ctxMerge{};
+
+    uint32_t* m_ranks = nullptr;
+    uint32_t* m_lists[kClasses] = {};
+    uint64_t m_listLength[kClasses] = {};
+
+    int sync() {
+        if (!m_dry) {
+            SADCS_HIP(hipStreamSynchronize(m_stream));
+        }
+        return SADCS_OK;
+    }
+
+    void releaseAll() {
+        for (int d = 0; d < kClasses; d++) {
+            m_arena.release(m_lists[d]);
+        }
+        m_arena.release(m_ranks);
+    }
+
+    static unsigned blocksFor(uint64_t count, unsigned threads = 256) {
+        return static_cast<unsigned>(std::max<uint64_t>(1, (count + threads - 1) / threads));
+    }
+
+    void fillWords(uint32_t* out, uint64_t count, uint32_t value) {
+        if (!m_dry && count) {
+            fillWordsKernel<<<std::min<unsigned>(blocksFor(count), 4096), 256, 0, m_stream>>>(out, count, value);
+        }
+    }
+
+    // ---- phase (a) ------------------------------------------------------------------------------
+
+    int rankSample() {
+        const uint64_t rankWords = static_cast<uint64_t>(Cover::pack(m_job.span)) + CoverSet<V>::kSize + 8;
+        SADCS_TRY(m_arena.allocate(m_ranks, rankWords));
+        SADCS_TRY(m_arena.allocate(m_lists[0], std::max<uint32_t>(m_samples, 1)));
+        m_listLength[0] = m_samples;
+        fillWords(m_ranks, rankWords, 0);
+
+        const uint32_t M = m_samples;
+        const uint32_t groupCapacity = M / 2 + m_segments + 1;
+        uint64_t* keys[2] = {};
+        uint32_t* values[2] = {};
+        uint32_t* groupIds[2] = {};
+        uint32_t* hist = nullptr;
+        Sum1* sortAggregates = nullptr;
+        MarkVec* markAggregates = nullptr;
+        uint32_t* counts = nullptr;
+        uint32_t* byteCounts = nullptr;
+
+        int status = SADCS_OK;
+        auto cleanup = [&]() {
+            m_arena.release(byteCounts);
+            m_arena.release(counts);
+            m_arena.release(markAggregates);
+            m_arena.release(sortAggregates);
+            m_arena.release(hist);
+            for (int i = 1; i >= 0; i--) {
+                m_arena.release(groupIds[i]);
+                m_arena.release(values[i]);
+                m_arena.release(keys[i]);
+            }
+        };
+
+        for (int i = 0; i < 2 && status == SADCS_OK; i++) {
+            status = m_arena.allocate(keys[i], M);
+            if (status == SADCS_OK) status = m_arena.allocate(values[i], M);
+            if (status == SADCS_OK) status = m_arena.allocate(groupIds[i], groupCapacity);
+        }
+        if (status == SADCS_OK) status = m_arena.allocate(hist, radixHistWords(M));
+        if (status == SADCS_OK) status = m_arena.allocate(sortAggregates, radixAggregateBytes(M) / sizeof(Sum1) + 1);
+        if (status == SADCS_OK) status = m_arena.allocate(markAggregates, scanScratchBytes<MarkVec, kMarkItems>(M) / sizeof(MarkVec) + 1);
+        if (status == SADCS_OK) status = m_arena.allocate(counts, 4);
+        if (status == SADCS_OK) status = m_arena.allocate(byteCounts, 256);
+        if (status != SADCS_OK || m_dry || M == 0) {
+            cleanup();
+            return status;
+        }
+
+        // The alphabet: dense codes 1..sigma for the bytes present.
+        fillWords(byteCounts, 256, 0);
+        byteHistogramKernel<Texts><<<std::min<unsigned>(blocksFor(m_job.span), 2048), 256, 0, m_stream>>>(m_job.text, m_job.span, m_texts, byteCounts);
+        SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords, byteCounts, 256 * sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
+        SADCS_HIP(hipStreamSynchronize(m_stream));
+        CodeTable codes{};
+        int sigma = 0;
+        for (int b = 0; b < 256; b++) {
+            if (m_ctx.hostWords[b]) {
+                codes.code[b] = static_cast<uint8_t>(++sigma);  // 256 wraps to 0 only if sigma = 256: see below
+            }
+        }
+        const int charBits = std::max(1, bitWidth(static_cast<uint64_t>(sigma)));
+        if (sigma == 256) {
+            // 9-bit codes do not fit a byte table; use the byte plus one computed in the key kernel.
+            for (int b = 0; b < 256; b++) {
+                codes.code[b] = static_cast<uint8_t>(b);
+            }
+        }
+
+        RankParams rp{};
+        rp.text = reinterpret_cast<const uint64_t*>(m_job.text);
+        rp.ranks = m_ranks;
+        rp.sortedSample = m_lists[0];
+        rp.samples = M;
+        rp.segments = m_segments;
+        rp.segmentTable = m_job.deviceSegments;
+
+        initGroupIdsKernel<<<blocksFor(m_segments), 256, 0, m_stream>>>(m_job.deviceSegments, m_segments, groupIds[0]);
+
+        SortBuffers<uint64_t> sb;
+        sb.keys[0] = keys[0];
+        sb.keys[1] = keys[1];
+        sb.values[0] = values[0];
+        sb.values[1] = values[1];
+        sb.hist = hist;
+        sb.aggregates = sortAggregates;
+
+        uint32_t u = M;
+        uint32_t groups = m_segments;
+        uint64_t depth = 0;
+        bool first = true;
+        int listBuffer = 1;  // where the list's t and ordinals are (values[x], keys[x] as words)
+        int gcur = 0;
+        const int rankBits = bitWidth(M);
+
+        while (u > 0) {
+            const int ordinalBits = groups > 1 ? bitWidth(groups - 1) : 0;
+            PassSpec spec{};
+            if (depth < static_cast<uint64_t>(V)) {
+                spec.doubling = false;
+                spec.offset = static_cast<uint32_t>(depth);
+                spec.charBits = sigma == 256 ? 9 : charBits;
+                spec.chars = std::max(1, (64 - ordinalBits) / spec.charBits);
+                spec.subkeyBits = spec.chars * spec.charBits;
+            } else {
+                const uint64_t h = depth / V * V;
+                if (h >= m_job.maxLength) {
+                    setError("sample ranking: groups left at depth %llu (internal error)", static_cast<unsigned long long>(depth));
+                    cleanup();
+                    return SADCS_FAILED;
+                }
+                spec.doubling = true;
+                spec.offset = static_cast<uint32_t>(h);
+                spec.subkeyBits = rankBits;
+            }
+
+            const int y = 1 - listBuffer;
+            if (sigma == 256 && !spec.doubling) {
+                makePassKeysKernel<V, Texts, true><<<blocksFor(u), 256, 0, m_stream>>>(
+                    rp, m_texts, spec, codes, first, values[listBuffer], reinterpret_cast<const uint32_t*>(keys[listBuffer]), u, keys[y], values[y]);
+            } else {
+                makePassKeysKernel<V, Texts, false><<<blocksFor(u), 256, 0, m_stream>>>(
+                    rp, m_texts, spec, codes, first, values[listBuffer], reinterpret_cast<const uint32_t*>(keys[listBuffer]), u, keys[y], values[y]);
+            }
+            sb.current = y;
+            radixSort<uint64_t>(m_stream, sb, u, 0, ordinalBits + spec.subkeyBits);
+            const int c = sb.current;
+
+            MarkState ms{};
+            ms.keys = keys[c];
+            ms.values = values[c];
+            ms.u = u;
+            ms.subkeyBits = spec.subkeyBits;
+            ms.groupIds = groupIds[gcur];
+            ms.ranks = m_ranks;
+            ms.sortedSample = m_lists[0];
+            ms.nextT = values[1 - c];
+            ms.nextOrdinal = reinterpret_cast<uint32_t*>(keys[1 - c]);
+            ms.nextGroupIds = groupIds[1 - gcur];
+            ms.counts = counts;
+            ms.samples = M;
+            scanPipeline<MarkVec, kMarkItems>(m_stream, u, MarkProducer{ms}, MarkConsumer<V>{ms}, markAggregates);
+            SADCS_HIP(hipGetLastError());
+            SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords, counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
+            SADCS_HIP(hipStreamSynchronize(m_stream));
+
+            u = m_ctx.hostWords[0];
+            groups = m_ctx.hostWords[1];
+            gcur = 1 - gcur;
+            listBuffer = 1 - c;
+            first = false;
+            if (spec.doubling) {
+                depth += spec.offset;
+                times.doublingRounds++;
+            } else {
+                depth += static_cast<uint64_t>(spec.chars);
+                times.characterPasses++;
+            }
+        }
+        cleanup();
+        return SADCS_OK;
+    }
+
+    // ---- phase (b) ------------------------------------------------------------------------------
+
+    int sortClasses() {
+        uint64_t largest = 0;
+        for (int d = 1; d < kClasses; d++) {
+            uint64_t size = 0;
+            for (const Segment& s : m_job.segments) {
+                size += classSizeOf<V>(s.end - s.start, d);
+            }
+            m_listLength[d] = size;
+            largest = std::max(largest, size);
+            SADCS_TRY(m_arena.allocate(m_lists[d], std::max<uint64_t>(size, 1)));
+        }
+
+        using Key = typename std::conditional<kBatch, uint64_t, uint32_t>::type;
+        Key* keys[2] = {};
+        Key* tmpKeys = nullptr;
+        uint32_t* tmpValues = nullptr;
+        uint32_t* hist = nullptr;
+        Sum1* aggregates = nullptr;
+        const uint64_t walk = largest + m_segments + static_cast<uint64_t>(m_listLength[0]);
+        int status = SADCS_OK;
+        auto cleanup = [&]() {
+            m_arena.release(aggregates);
+            m_arena.release(hist);
+            m_arena.release(tmpValues);
+            m_arena.release(tmpKeys);
+            m_arena.release(keys[1]);
+            m_arena.release(keys[0]);
+        };
+        status = m_arena.allocate(keys[0], std::max<uint64_t>(largest, 1));
+        if (status == SADCS_OK) status = m_arena.allocate(keys[1], std::max<uint64_t>(largest, 1));
+        if (kBatch && status == SADCS_OK) status = m_arena.allocate(tmpKeys, std::max<uint64_t>(largest, 1));
+        if (kBatch && status == SADCS_OK) status = m_arena.allocate(tmpValues, std::max<uint64_t>(largest, 1));
+        if (status == SADCS_OK) status = m_arena.allocate(hist, radixHistWords(walk));
+        if (status == SADCS_OK) status = m_arena.allocate(aggregates, radixAggregateBytes(walk) / sizeof(Sum1) + 1);
+        if (status != SADCS_OK || m_dry) {
+            cleanup();
+            return status;
+        }
+
+        const int segmentBits = m_segments > 1 ? bitWidth(m_segments - 1) : 0;
+        int prev = 0;  // keys[prev]: the previous class's keys
+        for (int d = 1; d < kClasses; d++) {
+            if (m_listLength[d] == 0) {
+                continue;
+            }
+            const bool fromText = (d - 1) % 4 == 0;
+            ClassWalkSource<V, Texts, Key> source{};
+            source.bytes = m_job.text;
+            source.previous = m_lists[d - 1];
+            source.previousKeys = fromText ? nullptr : keys[prev];
+            source.previousCount = static_cast<uint32_t>(m_listLength[d - 1]);
+            source.segmentTable = m_job.deviceSegments;
+            source.segments = m_segments;
+            source.n = m_job.span;
+            source.texts = m_texts;
+            source.d = d;
+            const uint32_t count = static_cast<uint32_t>(m_listLength[d - 1]) + m_segments;
+            const int next = 1 - prev;
+
+            SortBuffers<Key> sb;
+            sb.hist = hist;
+            sb.aggregates = aggregates;
+            const int extraPasses = kBatch ? (segmentBits + kRadixBits - 1) / kRadixBits : 0;
+            const int passes = 1 + extraPasses;
+            // Buffer 1 is the destination (next keys, class d); buffer 0 the temporaries.
+            sb.keys[0] = kBatch ? tmpKeys : keys[prev];
+            sb.values[0] = tmpValues;
+            sb.keys[1] = keys[next];
+            sb.values[1] = m_lists[d];
+            sb.current = passes % 2 == 1 ? 0 : 1;
+            radixPass<Key>(m_stream, sb, source, count, 24, 8, static_cast<uint32_t>(m_listLength[d]));
+            for (int e = 0; e < extraPasses; e++) {
+                const int shift = 32 + e * kRadixBits;
+                const int width = std::min(kRadixBits, 32 + segmentBits - shift);
+                radixPass<Key>(m_stream, sb, ArraySource<Key>{sb.keys[sb.current], sb.values[sb.current]}, static_cast<uint32_t>(m_listLength[d]), shift, width);
+            }
+            SADCS_HIP(hipGetLastError());
+            prev = next;
+        }
+        cleanup();
+        return SADCS_OK;
+    }
+
+    // ---- phase (c) ------------------------------------------------------------------------------
+
+    ListSet topLists() const {
+        ListSet l{};
+        l.count = kClasses;
+        uint32_t base = 0;
+        for (int d = 0; d < kClasses; d++) {
+            l.ptr[d] = m_lists[d];
+            l.length[d] = static_cast<uint32_t>(m_listLength[d]);
+            l.indexBase[d] = base;
+            base += l.length[d];
+        }
+        l.indexBase[kClasses] = base;
+        return l;
+    }
+
+    template <int kFlags>
+    int launchTiles(const ListSet& lists, const uint32_t* rows, uint32_t tiles, const TileOut& out, uint32_t* errors) {
+        const uint32_t capacity = mergeCapacity<V>(m_ctx.params.g, m_ctx.params.k);
+        const size_t shared = mergeSharedBytes<V, kBatch>(capacity);
+        if (m_dry) {
+            return SADCS_OK;
+        }
+        constexpr bool kTop = (kFlags & kOutIndex) == 0;
+        auto kernel = mergeTilesKernel<V, Texts, kFlags>;
+        SADCS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shared)));
+        if (kTop && m_ctx.kernelTimer[0]) {
+            SADCS_HIP(hipEventRecord(m_ctx.kernelTimer[0], m_stream));
+        }
+        kernel<<<tiles, kMergeThreads, shared, m_stream>>>(m_ctxMerge, m_texts, lists, rows, capacity, out, errors);
+        SADCS_HIP(hipGetLastError());
+        if (kTop && m_ctx.kernelTimer[0]) {
+            SADCS_HIP(hipEventRecord(m_ctx.kernelTimer[1], m_stream));
+            m_timedMerge = true;
+            times.mergeElements = lists.indexBase[lists.count];
+        }
+        return SADCS_OK;
+    }
+
+    bool m_timedMerge = false;
+
+    // Rows of the tiles of one level: (tiles + 1) rows of lists.count + 1 words, allocated here.
+    int levelRows(const ListSet& lists, uint32_t*& rows, uint32_t& tiles, uint32_t* errors) {
+        const uint32_t g = m_ctx.params.g, k = m_ctx.params.k;
+        const uint32_t capacity = mergeCapacity<V>(g, k);
+        const int lc = lists.count;
+        const uint64_t total = lists.indexBase[lc];
+
+        if (total <= capacity) {
+            tiles = 1;
+            SADCS_TRY(m_arena.allocate(rows, 2 * (lc + 1)));
+            if (!m_dry) {
+                singleTileRowsKernel<<<1, 1, 0, m_stream>>>(lists, rows);
+            }
+            return SADCS_OK;
+        }
+
+        // Splitters: every g-th element of every list, sorted by the level below.
+        ListSet splitters{};
+        splitters.count = lc;
+        uint32_t base = 0;
+        for (int l = 0; l < lc; l++) {
+            splitters.length[l] = (lists.length[l] + g - 1) / g;
+            splitters.indexBase[l] = base;
+            base += splitters.length[l];
+        }
+        splitters.indexBase[lc] = base;
+        const uint32_t count = base;
+        const uint32_t boundaries = (count + k - 1) / k;
+        tiles = boundaries;
+
+        SADCS_TRY(m_arena.allocate(rows, static_cast<uint64_t>(boundaries + 1) * (lc + 1)));
+        uint32_t* positions = nullptr;
+        uint32_t* sorted = nullptr;
+        uint32_t* counts = nullptr;
+        Sum1* aggregates = nullptr;
+        int status = m_arena.allocate(positions, count);
+        if (status == SADCS_OK) status = m_arena.allocate(sorted, count);
+        if (status == SADCS_OK) status = m_arena.allocate(counts, static_cast<uint64_t>(boundaries) * lc);
+        if (status == SADCS_OK) status = m_arena.allocate(aggregates, scanScratchBytes<Sum1, kSumItems>(static_cast<uint64_t>(boundaries) * lc) / sizeof(Sum1) + 1);
+        for (int l = 0; l < lc && status == SADCS_OK; l++) {
+            splitters.ptr[l] = positions + splitters.indexBase[l];
+        }
+        if (status == SADCS_OK && !m_dry) {
+            gatherSplittersKernel<<<blocksFor(count), 256, 0, m_stream>>>(lists, splitters, g, count, positions);
+        }
+        if (status == SADCS_OK) {
+            TileOut childOut{};
+            childOut.index = sorted;
+            status = mergeLevel<kOutIndex>(splitters, childOut, errors);
+        }
+        if (status == SADCS_OK && !m_dry) {
+            countBoundarySplittersKernel<<<blocksFor(boundaries), 256, 0, m_stream>>>(splitters, sorted, count, k, boundaries, counts);
+            exclusiveSum(m_stream, counts, counts, boundaries * static_cast<uint32_t>(lc), aggregates);
+            boundaryRowsKernel<V, Texts><<<blocksFor(boundaries + 1ull), 256, 0, m_stream>>>(
+                m_ctxMerge, m_texts, lists, splitters, sorted, counts, g, k, boundaries, rows);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) {
+                setError("merge boundaries: %s", hipGetErrorString(e));
+                status = SADCS_FAILED;
+            }
+        }
+        m_arena.release(aggregates);
+        m_arena.release(counts);
+        m_arena.release(sorted);
+        m_arena.release(positions);
+        if (status != SADCS_OK) {
+            m_arena.release(rows);
+        }
+        return status;
+    }
+
+    template <int kFlags>
+    int mergeLevel(const ListSet& lists, const TileOut& out, uint32_t* errors) {
+        uint32_t* rows = nullptr;
+        uint32_t tiles = 0;
+        SADCS_TRY(levelRows(lists, rows, tiles, errors));
+        const int status = launchTiles<kFlags>(lists, rows, tiles, out, errors);
+        m_arena.release(rows);
+        return status;
+    }
+
+    int merge() {
+        m_ctxMerge.ranks = m_ranks;
+        const ListSet lists = topLists();
+        uint32_t* errors = nullptr;
+        uint32_t* firstRows = m_job.firstRows;
+        bool ownFirstRows = false;
+        SADCS_TRY(m_arena.allocate(errors, 1));
+        fillWords(errors, 1, 0);
+
+        if (m_job.wantBWT && !firstRows) {
+            SADCS_TRY(m_arena.allocate(firstRows, m_segments));
+            ownFirstRows = true;
+        }
+        if (m_job.wantBWT && !m_dry) {
+            firstSuffixRowsKernel<V, Texts><<<blocksFor(m_segments), 256, 0, m_stream>>>(
+                m_ctxMerge, m_texts, lists, m_segments, firstRows, nullptr);
+        }
+
+        ListSet mergeLists = lists;
+        int status = SADCS_OK;
+        uint32_t* rows = nullptr;
+        uint32_t tiles = 0;
+        status = levelRows(lists, rows, tiles, errors);
+
+        TileOut out{};
+        out.firstSuffixRow = firstRows;
+        out.bwtLength = static_cast<uint32_t>(m_job.totalLength);
+        uint32_t* ownSA = nullptr;
+        uint8_t* ownBWT = nullptr;
+
+        if (status == SADCS_OK && m_job.wantSA && m_job.wantBWT) {
+            // Tile-major copy into the SA buffer, free the lists, then the BWT in their place.
+            out.sa = m_job.sa;
+            if (!out.sa) {
+                status = m_arena.allocate(ownSA, m_job.span);
+                out.sa = ownSA;
+            }
+            if (status == SADCS_OK && !m_dry) {
+                tileMajorCopyKernel<<<tiles, 256, 0, m_stream>>>(lists, rows, out.sa);
+            }
+            // Stream order keeps the copy's reads of the lists ahead of any write to their memory.
+            for (int d = 0; d < kClasses; d++) {
+                m_arena.release(m_lists[d]);
+            }
+            out.bwt = m_job.bwt;
+            if (status == SADCS_OK && !out.bwt) {
+                status = m_arena.allocate(ownBWT, m_job.span);
+                out.bwt = ownBWT;
+            }
+            if (status == SADCS_OK && !m_dry) {
+                firstBwtBytesKernel<<<blocksFor(m_segments), 256, 0, m_stream>>>(m_job.text, m_job.deviceSegments, m_segments, out.bwt);
+            }
+            if (status == SADCS_OK) {
+                status = launchTiles<kOutSA | kOutBWT | kInPlace>(mergeLists, rows, tiles, out, errors);
+            }
+        } else if (status == SADCS_OK && m_job.wantSA) {
+            out.sa = m_job.sa;
+            if (!out.sa) {
+                status = m_arena.allocate(ownSA, m_job.span);
+                out.sa = ownSA;
+            }
+            if (status == SADCS_OK) {
+                status = launchTiles<kOutSA>(mergeLists, rows, tiles, out, errors);
+            }
+        } else if (status == SADCS_OK && m_job.wantBWT) {
+            out.bwt = m_job.bwt;
+            if (!out.bwt) {
+                status = m_arena.allocate(ownBWT, m_job.span);
+                out.bwt = ownBWT;
+            }
+            if (status == SADCS_OK && !m_dry) {
+                firstBwtBytesKernel<<<blocksFor(m_segments), 256, 0, m_stream>>>(m_job.text, m_job.deviceSegments, m_segments, out.bwt);
+            }
+            if (status == SADCS_OK) {
+                status = launchTiles<kOutBWT>(mergeLists, rows, tiles, out, errors);
+            }
+        }
+
+        if (status == SADCS_OK && !m_dry) {
+            SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords + 8, errors, sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
+            SADCS_HIP(hipStreamSynchronize(m_stream));
+            if (m_ctx.hostWords[8] != 0) {
+                setError("merge: inconsistent tile layout (internal error, code %u)", m_ctx.hostWords[8]);
+                status = SADCS_FAILED;
+            }
+            if (m_timedMerge) {
+                float ms = 0;
+                SADCS_HIP(hipEventElapsedTime(&ms, m_ctx.kernelTimer[0], m_ctx.kernelTimer[1]));
+                times.mergeKernel = ms * 1e-3;
+            }
+        }
+        m_outSA = ownSA;
+        m_outBWT = ownBWT;
+        m_arena.release(rows);
+        if (ownFirstRows) {
+            m_firstRowsOwned = firstRows;
+        }
+        m_arena.release(errors);
+        return status;
+    }
+
+ public:
+    // Outputs the pipeline allocated itself (host-memory calls), valid until destruction.
+    uint32_t* m_outSA = nullptr;
+    uint8_t* m_outBWT = nullptr;
+    uint32_t* m_firstRowsOwned = nullptr;
+
+    void releaseOutputs() {
+        m_arena.release(m_outSA);
+        m_arena.release(m_outBWT);
+        m_arena.release(m_firstRowsOwned);
+    }
+};
+
+}  // namespace sadcs
+
+// ==== host side ==================================================================================
+
+namespace sadcs {
+
+enum class Mode { SA, BWT, Both, RankOnly };
+
+// What a job needs besides the pipeline: the text buffer and the segment tables.
+struct Layout {
+    uint32_t span = 0;
+    std::vector<Segment> segments;
+    std::vector<uint32_t> blockSegment;  // batch only
+    uint32_t maxLength = 0;
+    uint64_t totalLength = 0;
+};
+
+template <int V>
+uint32_t samplesOf(uint64_t n) {
+    return static_cast<uint32_t>(DifferenceCover<V>::pack(static_cast<int64_t>(n)));
+}
+
+// Texts at aligned offsets; lengths >= 1.
+template <int V>
+Layout makeLayout(const int64_t* lengths, int32_t count, bool batch) {
+    Layout layout;
+    uint64_t start = 0, rows = 0, ranks = 0;
+    for (int32_t i = 0; i < count; i++) {
+        const uint64_t n = static_cast<uint64_t>(lengths[i]);
+        Segment s{};
+        s.start = static_cast<uint32_t>(start);
+        s.end = static_cast<uint32_t>(start + n);
+        s.rankBase = static_cast<uint32_t>(ranks);
+        s.rowBase = static_cast<uint32_t>(rows);
+        layout.segments.push_back(s);
+        layout.maxLength = std::max<uint32_t>(layout.maxLength, static_cast<uint32_t>(n));
+        rows += n;
+        ranks += samplesOf<V>(n);
+        start = batch ? (start + n + kSegmentAlign - 1) / kSegmentAlign * kSegmentAlign : start + n;
+    }
+    layout.totalLength = rows;
+    layout.span = count > 0 ? layout.segments.back().end : 0;
+    if (batch) {
+        const uint64_t blocks = (static_cast<uint64_t>(layout.span) >> kSegmentBlockLog) + 2;
+        layout.blockSegment.assign(blocks, 0);
+        for (int32_t i = 0; i < count; i++) {
+            const uint64_t first = layout.segments[i].start >> kSegmentBlockLog;
+            const uint64_t last = i + 1 < count ? (layout.segments[i + 1].start >> kSegmentBlockLog) : blocks;
+            for (uint64_t b = first; b < last; b++) {
+                layout.blockSegment[b] = static_cast<uint32_t>(i);
+            }
+        }
+    }
+    return layout;
+}
+
+// Device buffers of the job's inputs, allocated from the arena (the text only if not given).
+struct JobInputs {
+    uint8_t* text = nullptr;
+    Segment* segments = nullptr;
+    uint32_t* blockSegment = nullptr;
+    uint32_t* firstRows = nullptr;
+};
+
+int allocateInputs(Arena& arena, const Layout& layout, bool ownText, JobInputs& in) {
+    if (ownText) {
+        SADCS_TRY(arena.allocate(in.text, static_cast<uint64_t>(layout.span) + kTextPadding + 8));
+    }
+    SADCS_TRY(arena.allocate(in.segments, layout.segments.size()));
+    SADCS_TRY(arena.allocate(in.firstRows, layout.segments.size()));
+    if (!layout.blockSegment.empty()) {
+        SADCS_TRY(arena.allocate(in.blockSegment, layout.blockSegment.size()));
+    }
+    return SADCS_OK;
+}
+
+// Runs the pipeline for one cover and text policy. In a dry-run arena only the allocation
+// sequence runs; arena.highWater() is then the plan.
+template <int V, class Texts>
+int runPipeline(sadcs_ctx& ctx, Job& job, Texts texts, PhaseTimes& times, uint32_t** saOut, uint8_t** bwtOut, std::unique_ptr<Pipeline<V, Texts>>& keep) {
+    keep.reset(new Pipeline<V, Texts>(ctx, job, texts));
+    const int status = keep->run();
+    times = keep->times;
+    if (saOut) {
+        *saOut = keep->m_outSA;
+    }
+    if (bwtOut) {
+        *bwtOut = keep->m_outBWT;
+    }
+    return status;
+}
+
+// The planned peak of a job of this layout and mode, in bytes (a dry run of the allocation sequence).
+template <int V>
+size_t planBytes(const sadcs_ctx& base, const Layout& layout, Mode mode, bool ownText, bool ownOutputs) {
+    sadcs_ctx ctx;
+    ctx.params = base.params;
+    ctx.arena.reset(nullptr, SIZE_MAX / 4, true);
+    JobInputs in;
+    if (allocateInputs(ctx.arena, layout, ownText, in) != SADCS_OK) {
+        return SIZE_MAX;
+    }
+    Job job;
+    job.text = in.text;
+    job.span = layout.span;
+    job.segments = layout.segments;
+    job.deviceSegments = in.segments;
+    job.blockSegment = in.blockSegment;
+    job.batch = !layout.blockSegment.empty();
+    job.rankOnly = mode == Mode::RankOnly;
+    job.wantSA = mode == Mode::SA || mode == Mode::Both;
+    job.wantBWT = mode == Mode::BWT || mode == Mode::Both;
+    job.firstRows = in.firstRows;
+    job.maxLength = layout.maxLength;
+    job.totalLength = layout.totalLength;
+    uint32_t* dummySA = ownOutputs ? nullptr : reinterpret_cast<uint32_t*>(uintptr_t(1) << 40);
+    uint8_t* dummyBWT = ownOutputs ? nullptr : reinterpret_cast<uint8_t*>(uintptr_t(1) << 41);
+    job.sa = job.wantSA ? dummySA : nullptr;
+    job.bwt = job.wantBWT ? dummyBWT : nullptr;
+    PhaseTimes times;
+    int status;
+    if (job.batch) {
+        std::unique_ptr<Pipeline<V, TextBatch>> keep;
+        status = runPipeline<V, TextBatch>(ctx, job, TextBatch{}, times, nullptr, nullptr, keep);
+    } else {
+        std::unique_ptr<Pipeline<V, SingleText>> keep;
+        status = runPipeline<V, SingleText>(ctx, job, SingleText{layout.span}, times, nullptr, nullptr, keep);
+    }
+    return status == SADCS_OK ? ctx.arena.highWater() : SIZE_MAX;
+}
+
+size_t planFor(const sadcs_ctx& base, const Layout& layout, Mode mode, bool ownText, bool ownOutputs) {
+    return base.params.cover == 128 ? planBytes<128>(base, layout, mode, ownText, ownOutputs)
+                                    : planBytes<32>(base, layout, mode, ownText, ownOutputs);
+}
+
+// The largest plan over the modes a context serves: SA, BWT, both.
+size_t contextPlan(const sadcs_ctx& base, int64_t capacity) {
+    if (capacity <= 1) {
+        return 1 << 20;
+    }
+    const int64_t lengths[1] = {capacity};
+    Layout layout = base.params.cover == 128 ? makeLayout<128>(lengths, 1, false) : makeLayout<32>(lengths, 1, false);
+    size_t peak = 0;
+    for (Mode mode : {Mode::SA, Mode::BWT, Mode::Both}) {
+        peak = std::max(peak, planFor(base, layout, mode, true, true));
+    }
+    return peak;
+}
+
+int queryDevice(sadcs_device& device) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) {
+        setError("no HIP device%s%s", e != hipSuccess ? ": " : "", e != hipSuccess ? hipGetErrorString(e) : "");
+        return SADCS_UNAVAILABLE;
+    }
+    hipDeviceProp_t properties;
+    if ((e = hipGetDeviceProperties(&properties, 0)) != hipSuccess) {
+        setError("hipGetDeviceProperties: %s", hipGetErrorString(e));
+        return SADCS_UNAVAILABLE;
+    }
+    size_t freeBytes = 0, totalBytes = 0;
+    if ((e = hipMemGetInfo(&freeBytes, &totalBytes)) != hipSuccess) {
+        setError("hipMemGetInfo: %s", hipGetErrorString(e));
+        return SADCS_UNAVAILABLE;
+    }
+    snprintf(device.name, sizeof device.name, "%s (%s)", properties.name, properties.gcnArchName);
+    device.compute_major = properties.major;
+    device.compute_minor = properties.minor;
+    device.total_bytes = static_cast<int64_t>(totalBytes);
+    device.free_bytes = static_cast<int64_t>(freeBytes);
+    return SADCS_OK;
+}
+
+bool isPinned(const void* host) {
+    hipPointerAttribute_t attributes;
+    if (hipPointerGetAttributes(&attributes, host) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attributes.type == hipMemoryTypeHost;
+}
+
+// Host to device through two pinned chunks, or directly when the host buffer is pinned.
+int upload(sadcs_ctx& ctx, void* device, const void* host, size_t bytes, bool& pinned) {
+    pinned = true;
+    if (bytes == 0) {
+        return SADCS_OK;
+    }
+    if (isPinned(host) || !ctx.staging[0]) {
+        pinned = ctx.staging[0] != nullptr || isPinned(host);
+        SADCS_HIP(hipMemcpyAsync(device, host, bytes, hipMemcpyHostToDevice, ctx.stream));
+        return SADCS_OK;
+    }
+    for (size_t k = 0, offset = 0; offset < bytes; k++, offset += kStagingChunk) {
+        const size_t length = std::min(kStagingChunk, bytes - offset);
+        const int b = static_cast<int>(k & 1);
+        SADCS_HIP(hipEventSynchronize(ctx.stagingDone[b]));
+        memcpy(ctx.staging[b], static_cast<const uint8_t*>(host) + offset, length);
+        SADCS_HIP(hipMemcpyAsync(static_cast<uint8_t*>(device) + offset, ctx.staging[b], length, hipMemcpyHostToDevice, ctx.stream));
+        SADCS_HIP(hipEventRecord(ctx.stagingDone[b], ctx.stream));
+    }
+    return SADCS_OK;
+}
+
+// Device to host after the stream's earlier work; returns with the data on the host.
+int download(sadcs_ctx& ctx, void* host, const void* device, size_t bytes) {
+    if (bytes == 0) {
+        return SADCS_OK;
+    }
+    if (isPinned(host) || !ctx.staging[0]) {
+        SADCS_HIP(hipMemcpyAsync(host, device, bytes, hipMemcpyDeviceToHost, ctx.stream));
+        SADCS_HIP(hipStreamSynchronize(ctx.stream));
+        return SADCS_OK;
+    }
+    const size_t chunks = (bytes + kStagingChunk - 1) / kStagingChunk;
+    auto issue = [&](size_t k) -> int {
+        const size_t offset = k * kStagingChunk;
+        const int b = static_cast<int>(k & 1);
+        SADCS_HIP(hipMemcpyAsync(ctx.staging[b], static_cast<const uint8_t*>(device) + offset,
+                                 std::min(kStagingChunk, bytes - offset), hipMemcpyDeviceToHost, ctx.stream));
+        SADCS_HIP(hipEventRecord(ctx.stagingDone[b], ctx.stream));
+        return SADCS_OK;
+    };
+    SADCS_TRY(issue(0));
+    for (size_t k = 0; k < chunks; k++) {
+        if (k + 1 < chunks) {
+            SADCS_TRY(issue(k + 1));
+        }
+        const int b = static_cast<int>(k & 1);
+        const size_t offset = k * kStagingChunk;
+        SADCS_HIP(hipEventSynchronize(ctx.stagingDone[b]));
+        memcpy(static_cast<uint8_t*>(host) + offset, ctx.staging[b], std::min(kStagingChunk, bytes - offset));
+    }
+    return SADCS_OK;
+}
+
+int createContext(size_t arenaBytes, const Params& params, int32_t capacity, sadcs_ctx*& out) {
+    out = nullptr;
+    sadcs_device device;
+    SADCS_TRY(queryDevice(device));
+    if (static_cast<int64_t>(arenaBytes) + kMemoryReserve > device.free_bytes) {
+        setError("needs %lld MiB of device memory plus %lld MiB reserve, %lld MiB free",
+                 static_cast<long long>(arenaBytes >> 20), static_cast<long long>(kMemoryReserve >> 20),
+                 static_cast<long long>(device.free_bytes >> 20));
+        return SADCS_NOT_ENOUGH_MEMORY;
+    }
+    std::unique_ptr<sadcs_ctx> ctx(new (std::nothrow) sadcs_ctx);
+    if (!ctx) {
+        setError("out of host memory");
+        return SADCS_FAILED;
+    }
+    ctx->params = params;
+    ctx->capacity = capacity;
+    SADCS_HIP(hipSetDevice(0));
+    SADCS_TRY(ensureTables());
+    SADCS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    SADCS_HIP(hipDeviceGetAttribute(&ctx->sms, hipDeviceAttributeMultiprocessorCount, 0));
+    ctx->arenaBytes = std::max<size_t>(arenaBytes, Arena::kAlign);
+    {
+        const hipError_t e = hipMalloc(&ctx->arenaBase, ctx->arenaBytes);
+        if (e != hipSuccess) {
+            setError("hipMalloc(%zu): %s", ctx->arenaBytes, hipGetErrorString(e));
+            (void)hipStreamDestroy(ctx->stream);
+            return statusOf(e);
+        }
+    }
+    if (hipHostMalloc(reinterpret_cast<void**>(&ctx->hostWords), 4096, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        setError("hipHostMalloc of readback words failed");
+        (void)hipFree(ctx->arenaBase);
+        (void)hipStreamDestroy(ctx->stream);
+        return SADCS_FAILED;
+    }
+    for (int i = 0; i < 2; i++) {
+        if (hipEventCreate(&ctx->kernelTimer[i]) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->kernelTimer[i] = nullptr;
+        }
+    }
+    if (!ctx->kernelTimer[0] || !ctx->kernelTimer[1]) {
+        ctx->kernelTimer[0] = nullptr;
+    }
+    for (int i = 0; i < 2; i++) {
+        if (hipHostMalloc(reinterpret_cast<void**>(&ctx->staging[i]), kStagingChunk, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->stagingDone[i], hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            for (int j = 0; j <= i; j++) {
+                if (ctx->staging[j]) {
+                    (void)hipHostFree(ctx->staging[j]);
+                    ctx->staging[j] = nullptr;
+                }
+            }
+            break;
+        }
+    }
+    out = ctx.release();
+    return SADCS_OK;
+}
+
+void destroyContext(sadcs_ctx* ctx) {
+    if (!ctx) {
+        return;
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < 2; i++) {
+        if (ctx->kernelTimer[i]) {
+            (void)hipEventDestroy(ctx->kernelTimer[i]);
+        }
+        if (ctx->stagingDone[i]) {
+            (void)hipEventDestroy(ctx->stagingDone[i]);
+        }
+        if (ctx->staging[i]) {
+            (void)hipHostFree(ctx->staging[i]);
+        }
+    }
+    if (ctx->hostWords) {
+        (void)hipHostFree(ctx->hostWords);
+    }
+    if (ctx->arenaBase) {
+        (void)hipFree(ctx->arenaBase);
+    }
+    if (ctx->stream) {
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+void fillStats(sadcs_stats* stats, const PhaseTimes& t, size_t deviceBytes, int cover) {
+    if (!stats) {
+        return;
+    }
+    stats->device_bytes = static_cast<int64_t>(deviceBytes);
+    stats->ranking_bytes = static_cast<int64_t>(t.rankingBytes);
+    stats->class_sort_bytes = static_cast<int64_t>(t.classBytes);
+    stats->merge_bytes = static_cast<int64_t>(t.mergeBytes);
+    stats->ranking_seconds = t.ranking;
+    stats->class_sort_seconds = t.classes;
+    stats->merge_seconds = t.merge;
+    stats->bwt_seconds = t.bwt;
+    stats->device_seconds = t.ranking + t.classes + t.merge + t.bwt;
+    stats->merge_kernel_seconds = t.mergeKernel;
+    stats->merge_kernel_elements = static_cast<int64_t>(t.mergeElements);
+    stats->character_passes = t.characterPasses;
+    stats->doubling_rounds = t.doublingRounds;
+    stats->cover = cover;
+}
+
+// A whole build of one layout on a context: texts either uploaded from the host (hostText) or
+// already on the device (deviceText, laid out as the layout says, zero padded).
+struct BuildRequest {
+    Mode mode = Mode::Both;
+    const uint8_t* hostText = nullptr;       // concatenated without gaps, lengths from the layout
+    const int64_t* hostLengths = nullptr;
+    const uint8_t* deviceText = nullptr;
+    uint32_t* deviceSA = nullptr;            // device outputs (device calls) or null
+    uint8_t* deviceBWT = nullptr;
+    int32_t* hostSA = nullptr;               // host outputs (host calls) or null
+    uint8_t* hostBWT = nullptr;
+    int32_t* hostPrimaries = nullptr;        // one per text
+    uint32_t* hostRanks = nullptr;           // rank-only
+};
+
+template <int V, class Texts>
+int buildWith(sadcs_ctx& ctx, const Layout& layout, const BuildRequest& rq, Texts texts, sadcs_stats* stats) {
+    const auto wallStart = std::chrono::steady_clock::now();
+    Arena& arena = ctx.arena;
+    arena.reset(ctx.arenaBase, ctx.arenaBytes, false);
+    const bool deviceCall = rq.deviceText != nullptr;
+    JobInputs in;
+    SADCS_TRY(allocateInputs(arena, layout, !deviceCall, in));
+    const uint8_t* text = deviceCall ? rq.deviceText : in.text;
+    const bool batch = !layout.blockSegment.empty();
+    if constexpr (Texts::kBatch) {
+        texts.blockSegment = in.blockSegment;
+        texts.segments = in.segments;
+    }
+
+    // Segment tables through the pinned readback words when they fit, else synchronously.
+    const size_t segmentBytes = layout.segments.size() * sizeof(Segment);
+    SADCS_HIP(hipMemcpy(in.segments, layout.segments.data(), segmentBytes, hipMemcpyHostToDevice));
+    if (batch) {
+        SADCS_HIP(hipMemcpy(in.blockSegment, layout.blockSegment.data(), layout.blockSegment.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+
+    double uploadSeconds = 0;
+    bool pinned = false;
+    if (!deviceCall) {
+        const auto start = std::chrono::steady_clock::now();
+        SADCS_HIP(hipMemsetAsync(in.text, 0, static_cast<size_t>(layout.span) + kTextPadding + 8, ctx.stream));
+        if (!batch) {
+            SADCS_TRY(upload(ctx, in.text, rq.hostText, layout.span, pinned));
+        } else {
+            const uint8_t* src = rq.hostText;
+            for (size_t i = 0; i < layout.segments.size(); i++) {
+                const size_t n = layout.segments[i].end - layout.segments[i].start;
+                SADCS_TRY(upload(ctx, in.text + layout.segments[i].start, src, n, pinned));
+                src += n;
+            }
+        }
+        SADCS_HIP(hipStreamSynchronize(ctx.stream));
+        uploadSeconds = secondsSince(start);
+    }
+
+    Job job;
+    job.text = text;
+    job.span = layout.span;
+    job.segments = layout.segments;
+    job.deviceSegments = in.segments;
+    job.blockSegment = in.blockSegment;
+    job.batch = batch;
+    job.rankOnly = rq.mode == Mode::RankOnly;
+    job.wantSA = rq.mode == Mode::SA || rq.mode == Mode::Both;
+    job.wantBWT = rq.mode == Mode::BWT || rq.mode == Mode::Both;
+    job.sa = job.wantSA ? rq.deviceSA : nullptr;
+    job.bwt = job.wantBWT ? rq.deviceBWT : nullptr;
+    job.firstRows = in.firstRows;
+    job.maxLength = layout.maxLength;
+    job.totalLength = layout.totalLength;
+
+    PhaseTimes times;
+    uint32_t* outSA = nullptr;
+    uint8_t* outBWT = nullptr;
+    std::unique_ptr<Pipeline<V, Texts>> keep;
+    const int status = runPipeline<V, Texts>(ctx, job, texts, times, &outSA, &outBWT, keep);
+    size_t deviceBytes = arena.highWater();
+    if (deviceCall) {
+        deviceBytes += static_cast<size_t>(layout.span) + kTextPadding;
+        deviceBytes += job.wantSA ? static_cast<size_t>(layout.totalLength) * 4 : 0;
+        deviceBytes += job.wantBWT ? static_cast<size_t>(layout.totalLength) : 0;
+    }
+    fillStats(stats, times, deviceBytes, V);
+    if (stats) {
+        stats->upload_seconds = uploadSeconds;
+        stats->pinned = pinned ? 1 : 0;
+    }
+    if (status != SADCS_OK) {
+        return status;
+    }
+
+    const auto downloadStart = std::chrono::steady_clock::now();
+    const size_t count = layout.segments.size();
+    if (job.wantBWT && rq.hostPrimaries) {
+        SADCS_TRY(download(ctx, ctx.hostWords + 16, in.firstRows, count <= 1000 ? count * sizeof(uint32_t) : 0));
+        std::vector<uint32_t> rows;
+        const uint32_t* firstRows = ctx.hostWords + 16;
+        if (count > 1000) {
+            rows.resize(count);
+            SADCS_HIP(hipMemcpy(rows.data(), in.firstRows, count * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            firstRows = rows.data();
+        }
+        for (size_t i = 0; i < count; i++) {
+            rq.hostPrimaries[i] = static_cast<int32_t>(firstRows[i]) + 1;
+        }
+    }
+    if (job.rankOnly && rq.hostRanks) {
+        SADCS_TRY(download(ctx, rq.hostRanks, keep->ranks(), sizeof(uint32_t) * samplesOf<V>(layout.span)));
+    }
+    if (!deviceCall) {
+        if (job.wantSA && rq.hostSA) {
+            SADCS_TRY(download(ctx, rq.hostSA, outSA, static_cast<size_t>(layout.totalLength) * 4));
+        }
+        if (job.wantBWT && rq.hostBWT) {
+            SADCS_TRY(download(ctx, rq.hostBWT, outBWT, static_cast<size_t>(layout.totalLength)));
+        }
+    }
+    SADCS_HIP(hipStreamSynchronize(ctx.stream));
+    if (stats) {
+        stats->download_seconds = secondsSince(downloadStart);
+        (void)wallStart;
+    }
+    return SADCS_OK;
+}
+
+int build(sadcs_ctx& ctx, const Layout& layout, const BuildRequest& rq, sadcs_stats* stats) {
+    const bool batch = !layout.blockSegment.empty();
+    if (ctx.params.cover == 128) {
+        return batch ? buildWith<128, TextBatch>(ctx, layout, rq, TextBatch{}, stats)
+                     : buildWith<128, SingleText>(ctx, layout, rq, SingleText{layout.span}, stats);
+    }
+    return batch ? buildWith<32, TextBatch>(ctx, layout, rq, TextBatch{}, stats)
+                 : buildWith<32, SingleText>(ctx, layout, rq, SingleText{layout.span}, stats);
+}
+
+Layout layoutFor(const Params& params, const int64_t* lengths, int32_t count, bool batch) {
+    return params.cover == 128 ? makeLayout<128>(lengths, count, batch) : makeLayout<32>(lengths, count, batch);
+}
+
+bool validParams(const Params& p) {
+    if (p.cover != 32 && p.cover != 128) {
+        setError("cover must be 32 or 128, not %d", p.cover);
+        return false;
+    }
+    if (p.g < 2 || p.k < 1 || p.g > 4096 || p.k > 4096) {
+        setError("invalid merge parameters (splitter spacing %u, tile splitters %u)", p.g, p.k);
+        return false;
+    }
+    const uint32_t capacity = p.cover == 128 ? mergeCapacity<128>(p.g, p.k) : mergeCapacity<32>(p.g, p.k);
+    const size_t shared = p.cover == 128 ? mergeSharedBytes<128, true>(capacity) : mergeSharedBytes<32, true>(capacity);
+    if (capacity > 65535 || shared > 160 * 1024 - 1024) {
+        setError("merge tiles of up to %u elements need %zu bytes of LDS, more than a workgroup has", capacity, shared);
+        return false;
+    }
+    return true;
+}
+
+// Builds with the given context, or a temporary one sized for this layout.
+int buildOn(sadcs_ctx* context, const Params& params, const Layout& layout, const BuildRequest& rq, sadcs_stats* stats) {
+    if (context) {
+        const bool deviceCall = rq.deviceText != nullptr;
+        const size_t need = planFor(*context, layout, rq.mode, !deviceCall, true);
+        if (need > context->arenaBytes) {
+            setError("input needs %zu bytes of the context's %zu", need, context->arenaBytes);
+            return SADCS_NOT_ENOUGH_MEMORY;
+        }
+        return build(*context, layout, rq, stats);
+    }
+    const auto start = std::chrono::steady_clock::now();
+    sadcs_ctx planner;
+    planner.params = params;
+    const size_t need = planFor(planner, layout, rq.mode, rq.deviceText == nullptr, true);
+    if (need == SIZE_MAX) {
+        setError("planning the device memory failed");
+        return SADCS_FAILED;
+    }
+    sadcs_ctx* ctx = nullptr;
+    SADCS_TRY(createContext(need, params, 0, ctx));
+    const double setup = secondsSince(start);
+    const int status = build(*ctx, layout, rq, stats);
+    if (stats) {
+        stats->setup_seconds += setup;
+    }
+    destroyContext(ctx);
+    return status;
+}
+
+}  // namespace sadcs
+
+// ==== C ABI ======================================================================================
+
+using namespace sadcs;
+
+extern "C" {
+
+int sadcs_hip_query(sadcs_device* device) {
+    tLastError[0] = '\0';
+    sadcs_device local;
+    return queryDevice(device ? *device : local);
+}
+
+int64_t sadcs_hip_sample_size(int32_t n, int32_t cover) {
+    if (n <= 0) {
+        return 0;
+    }
+    return cover == 128 ? DifferenceCover<128>::pack(n) : DifferenceCover<32>::pack(n);
+}
+
+int64_t sadcs_hip_required_bytes(int32_t n) {
+    if (n <= 0) {
+        return 0;
+    }
+    sadcs_ctx planner;
+    const size_t plan = contextPlan(planner, n);
+    return plan == SIZE_MAX ? -1 : static_cast<int64_t>(plan);
+}
+
+int sadcs_hip_ctx_create(int32_t capacity, const sadcs_params* params, sadcs_ctx** context) {
+    tLastError[0] = '\0';
+    if (!context) {
+        setError("null context pointer");
+        return SADCS_FAILED;
+    }
+    *context = nullptr;
+    Params p;
+    p.apply(params);
+    if (!validParams(p)) {
+        return SADCS_FAILED;
+    }
+    sadcs_device device;
+    SADCS_TRY(queryDevice(device));
+    sadcs_ctx planner;
+    planner.params = p;
+    const size_t plan = contextPlan(planner, std::max(capacity, 0));
+    if (plan == SIZE_MAX) {
+        setError("planning the device memory failed");
+        return SADCS_FAILED;
+    }
+    return createContext(plan, p, std::max(capacity, 0), *context);
+}
+
+void sadcs_hip_ctx_destroy(sadcs_ctx* context) {
+    destroyContext(context);
+}
+
+static int smallCase(const uint8_t* text, int32_t n, int32_t* sa, uint8_t* bwt, int32_t* primary) {
+    if (n == 1) {
+        if (sa) {
+            sa[0] = 0;
+        }
+        if (bwt) {
+            bwt[0] = text[0];
+        }
+        if (primary) {
+            *primary = 1;
+        }
+    } else if (primary) {
+        *primary = 0;
+    }
+    return SADCS_OK;
+}
+
+int sadcs_hip_build(sadcs_ctx* context, const uint8_t* text, int32_t n, int32_t* sa, uint8_t* bwt, int32_t* primary, sadcs_stats* stats) {
+    tLastError[0] = '\0';
+    if (stats) {
+        *stats = sadcs_stats{};
+    }
+    if (primary) {
+        *primary = n > 0 ? -1 : 0;
+    }
+    if (n <= 1) {
+        sadcs_device device;
+        const int status = context ? SADCS_OK : queryDevice(device);
+        return status == SADCS_OK ? smallCase(text, n, sa, bwt, primary) : status;
+    }
+    if (context && n > context->capacity) {
+        setError("input of %d bytes exceeds the context's capacity of %d bytes", n, context->capacity);
+        return SADCS_NOT_ENOUGH_MEMORY;
+    }
+    if (!sa && !bwt) {
+        return SADCS_OK;
+    }
+    Params params = context ? context->params : Params{};
+    const int64_t lengths[1] = {n};
+    const Layout layout = layoutFor(params, lengths, 1, false);
+    BuildRequest rq;
+    rq.mode = sa && bwt ? Mode::Both : sa ? Mode::SA : Mode::BWT;
+    rq.hostText = text;
+    rq.hostSA = sa;
+    rq.hostBWT = bwt;
+    int32_t first = -1;
+    rq.hostPrimaries = &first;
+    const int status = buildOn(context, params, layout, rq, stats);
+    if (status == SADCS_OK && primary && bwt) {
+        *primary = first;
+    }
+    return status;
+}
+
+int sadcs_hip_build_device(sadcs_ctx* context, const void* d_text, int32_t n, void* d_sa, void* d_bwt, int32_t* primary, sadcs_stats* stats) {
+    tLastError[0] = '\0';
+    if (stats) {
+        *stats = sadcs_stats{};
+    }
+    if (primary) {
+        *primary = n > 0 ? -1 : 0;
+    }
+    if (n <= 0 || (!d_sa && !d_bwt)) {
+        return SADCS_OK;
+    }
+    if (!d_text || (reinterpret_cast<uintptr_t>(d_text) & 7) != 0) {
+        setError("the device text must be 8-byte aligned");
+        return SADCS_FAILED;
+    }
+    if (context && n > context->capacity) {
+        setError("input of %d bytes exceeds the context's capacity of %d bytes", n, context->capacity);
+        return SADCS_NOT_ENOUGH_MEMORY;
+    }
+    if (n == 1) {
+        uint8_t c = 0;
+        SADCS_HIP(hipMemcpy(&c, d_text, 1, hipMemcpyDeviceToHost));
+        int32_t zero = 0;
+        if (d_sa) {
+            SADCS_HIP(hipMemcpy(d_sa, &zero, 4, hipMemcpyHostToDevice));
+        }
+        if (d_bwt) {
+            SADCS_HIP(hipMemcpy(d_bwt, &c, 1, hipMemcpyHostToDevice));
+        }
+        if (primary) {
+            *primary = 1;
+        }
+        return SADCS_OK;
+    }
+    Params params = context ? context->params : Params{};
+    const int64_t lengths[1] = {n};
+    const Layout layout = layoutFor(params, lengths, 1, false);
+    BuildRequest rq;
+    rq.mode = d_sa && d_bwt ? Mode::Both : d_sa ? Mode::SA : Mode::BWT;
+    rq.deviceText = static_cast<const uint8_t*>(d_text);
+    rq.deviceSA = static_cast<uint32_t*>(d_sa);
+    rq.deviceBWT = static_cast<uint8_t*>(d_bwt);
+    int32_t first = -1;
+    rq.hostPrimaries = &first;
+    const int status = buildOn(context, params, layout, rq, stats);
+    if (status == SADCS_OK && primary && d_bwt) {
+        *primary = first;
+    }
+    return status;
+}
+
+int sadcs_hip_rank_sample(const uint8_t* text, int32_t n, uint32_t* ranks, sadcs_stats* stats) {
+    tLastError[0] = '\0';
+    if (stats) {
+        *stats = sadcs_stats{};
+    }
+    if (n <= 0) {
+        sadcs_device device;
+        return queryDevice(device);
+    }
+    Params params;
+    const int64_t lengths[1] = {n};
+    const Layout layout = layoutFor(params, lengths, 1, false);
+    BuildRequest rq;
+    rq.mode = Mode::RankOnly;
+    rq.hostText = text;
+    rq.hostRanks = ranks;
+    return buildOn(nullptr, params, layout, rq, stats);
+}
+
+void* sadcs_hip_alloc_pinned(size_t bytes) {
+    void* pointer = nullptr;
+    if (hipHostMalloc(&pointer, std::max<size_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return pointer;
+}
+
+void sadcs_hip_free_pinned(void* pointer) {
+    if (pointer) {
+        (void)hipHostFree(pointer);
+    }
+}
+
+const char* sadcs_hip_last_error(void) {
+    return tLastError;
+}
+
+int sadcs_hip_batch_build(sadcs_ctx* context, const uint8_t* texts, const int64_t* lengths, int32_t count, int32_t* sa, uint8_t* bwt, int32_t* primaries, sadcs_stats* stats) {
+    tLastError[0] = '\0';
+    if (stats) {
+        *stats = sadcs_stats{};
+    }
+    if (count <= 0 || (!sa && !bwt)) {
+        return SADCS_OK;
+    }
+    uint64_t total = 0;
+    for (int32_t i = 0; i < count; i++) {
+        if (lengths[i] < 1 || lengths[i] > INT32_MAX) {
+            setError("text %d has length %lld; batched texts need 1 to 2^31-1 bytes", i, static_cast<long long>(lengths[i]));
+            return SADCS_FAILED;
+        }
+        total += static_cast<uint64_t>(lengths[i]) + kSegmentAlign;
+    }
+    if (total >= (uint64_t(1) << 31)) {
+        setError("a batch holds less than 2 GiB of text");
+        return SADCS_NOT_ENOUGH_MEMORY;
+    }
+    Params params = context ? context->params : Params{};
+    const Layout layout = layoutFor(params, lengths, count, true);
+    BuildRequest rq;
+    rq.mode = sa && bwt ? Mode::Both : sa ? Mode::SA : Mode::BWT;
+    rq.hostText = texts;
+    rq.hostLengths = lengths;
+    rq.hostSA = sa;
+    rq.hostBWT = bwt;
+    std::vector<int32_t> first(static_cast<size_t>(count), -1);
+    rq.hostPrimaries = first.data();
+    const int status = buildOn(context, params, layout, rq, stats);
+    if (status == SADCS_OK && primaries && bwt) {
+        std::copy(first.begin(), first.end(), primaries);
+    }
+    return status;
+}
+
+int64_t sadcs_hip_batch_required_bytes(const int64_t* lengths, int32_t count) {
+    if (count <= 0) {
+        return 0;
+    }
+    Params params;
+    const Layout layout = layoutFor(params, lengths, count, true);
+    sadcs_ctx planner;
+    const size_t plan = planFor(planner, layout, Mode::Both, true, true);
+    return plan == SIZE_MAX ? -1 : static_cast<int64_t>(plan);
+}
+
+}  // extern "C"

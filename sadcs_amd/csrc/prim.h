@@ -1,0 +1,394 @@
+/*
+ * prim.h - Device-wide primitives written for gfx950: a scan pipeline and an LSD radix sort.
+ *
+ * Both are reduce-then-scan: a tile kernel reduces, one block scans the tile aggregates, a second
+ * tile kernel scans with the tile's prefix. Nothing waits on another block, so there is no
+ * forward-progress assumption between workgroups.
+ *
+ * Scan pipeline. A Producer maps item i to a Vec of 32-bit fields (sum or max per field); a
+ * Consumer receives the item and its exclusive prefix. Fusing the per-item work into the
+ * consumer is how the phases mark groups, store ranks and compact lists in one pass.
+ *
+ * Radix sort. Stable LSD over a bit range with RB-bit digits. A tile of THREADS * ITEMS keys
+ * is ranked with ballots: the lanes holding the same digit are found with RB ballots, the
+ * rank among them is a masked popcount (mbcnt), and one leader lane per digit bumps the wave's
+ * counter. Keys are then placed in LDS in tile order and written out in runs per digit.
+ * The first pass may read from a Source functor that filters and computes keys, so a
+ * stable partition of a derived sequence (the class sorts) is one sort pass.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdint>
+#include "dev_common.h"
+
+namespace sadcs {
+
+inline uint32_t ceilDiv(uint64_t a, uint64_t b) {
+    return static_cast<uint32_t>((a + b - 1) / b);
+}
+
+// ==== scan pipeline ==============================================================================
+
+constexpr int kScanThreads = 256;
+constexpr int kScanWaves = kScanThreads / kWave;
+
+template <class VecT, int ITEMS, class Producer>
+__global__ __launch_bounds__(kScanThreads) void scanReduceKernel(Producer produce, uint32_t count, VecT* __restrict__ aggregates) {
+    __shared__ VecT waveTotals[kScanWaves];
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = threadIdx.x % kWave;
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanThreads * ITEMS + wave * ITEMS * kWave + lane;
+    VecT sum = VecT::zero();
+
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const uint64_t i = base + static_cast<uint64_t>(j) * kWave;
+        if (i < count) {
+            sum = VecT::combine(sum, produce(static_cast<uint32_t>(i)));
+        }
+    }
+
+    sum = VecT::lane(VecT::waveInclusive(sum), kWave - 1);
+    if (lane == 0) {
+        waveTotals[wave] = sum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        VecT total = VecT::zero();
+        for (int w = 0; w < kScanWaves; w++) {
+            total = VecT::combine(total, waveTotals[w]);
+        }
+        aggregates[blockIdx.x] = total;
+    }
+}
+
+// Exclusive scan of count aggregates in place by one block, looping; the total goes to aggregates[count].
+template <class VecT>
+__global__ __launch_bounds__(kScanThreads) void scanAggregatesKernel(VecT* __restrict__ aggregates, uint32_t count) {
+    __shared__ VecT waveTotals[kScanWaves];
+    __shared__ VecT carryShared;
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = threadIdx.x % kWave;
+
+    if (threadIdx.x == 0) {
+        carryShared = VecT::zero();
+    }
+    __syncthreads();
+
+    for (uint32_t chunk = 0; chunk < count; chunk += kScanThreads) {
+        const uint32_t i = chunk + threadIdx.x;
+        const VecT x = i < count ? aggregates[i] : VecT::zero();
+        const VecT inclusive = VecT::waveInclusive(x);
+        if (lane == kWave - 1) {
+            waveTotals[wave] = inclusive;
+        }
+        __syncthreads();
+        VecT prefix = carryShared;
+        for (uint32_t w = 0; w < wave; w++) {
+            prefix = VecT::combine(prefix, waveTotals[w]);
+        }
+        const VecT exclusive = VecT::combine(prefix, VecT::shiftUp(inclusive));
+        if (i < count) {
+            aggregates[i] = exclusive;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            VecT c = carryShared;
+            for (int w = 0; w < kScanWaves; w++) {
+                c = VecT::combine(c, waveTotals[w]);
+            }
+            carryShared = c;
+        }
+        __syncthreads();
+    }
+
+    if (threadIdx.x == 0) {
+        aggregates[count] = carryShared;
+    }
+}
+
+template <class VecT, int ITEMS, class Producer, class Consumer>
+__global__ __launch_bounds__(kScanThreads) void scanDownsweepKernel(Producer produce, Consumer consume, uint32_t count, const VecT* __restrict__ aggregates) {
+    __shared__ VecT waveTotals[kScanWaves];
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = threadIdx.x % kWave;
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanThreads * ITEMS + wave * ITEMS * kWave + lane;
+    VecT items[ITEMS];
+    VecT exclusive[ITEMS];
+    VecT carry = VecT::zero();
+
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const uint64_t i = base + static_cast<uint64_t>(j) * kWave;
+        items[j] = i < count ? produce(static_cast<uint32_t>(i)) : VecT::zero();
+        const VecT inclusive = VecT::waveInclusive(items[j]);
+        exclusive[j] = VecT::combine(carry, VecT::shiftUp(inclusive));
+        carry = VecT::combine(carry, VecT::lane(inclusive, kWave - 1));
+    }
+
+    if (lane == 0) {
+        waveTotals[wave] = carry;
+    }
+    __syncthreads();
+
+    VecT prefix = aggregates[blockIdx.x];
+    for (uint32_t w = 0; w < wave; w++) {
+        prefix = VecT::combine(prefix, waveTotals[w]);
+    }
+
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const uint64_t i = base + static_cast<uint64_t>(j) * kWave;
+        if (i < count) {
+            consume(static_cast<uint32_t>(i), items[j], VecT::combine(prefix, exclusive[j]));
+        }
+    }
+}
+
+// Scratch for a scan over up to `count` items: tile aggregates plus the total.
+template <class VecT, int ITEMS>
+inline size_t scanScratchBytes(uint64_t count) {
+    const uint64_t tiles = std::max<uint64_t>(1, ceilDiv(count, uint64_t(kScanThreads) * ITEMS));
+    return (tiles + 1) * sizeof(VecT);
+}
+
+// Runs produce/consume over [0, count). aggregates holds scanScratchBytes; afterwards
+// aggregates[tiles] is the total.
+template <class VecT, int ITEMS, class Producer, class Consumer>
+inline void scanPipeline(hipStream_t stream, uint32_t count, Producer produce, Consumer consume, VecT* aggregates) {
+    const uint32_t tiles = std::max<uint32_t>(1, ceilDiv(count, uint64_t(kScanThreads) * ITEMS));
+    scanReduceKernel<VecT, ITEMS><<<tiles, kScanThreads, 0, stream>>>(produce, count, aggregates);
+    scanAggregatesKernel<VecT><<<1, kScanThreads, 0, stream>>>(aggregates, tiles);
+    scanDownsweepKernel<VecT, ITEMS><<<tiles, kScanThreads, 0, stream>>>(produce, consume, count, aggregates);
+}
+
+using Sum1 = Vec<1, 0u>;
+
+struct LoadU32 {
+    const uint32_t* in;  // may be the output (in-place scans)
+    __device__ Sum1 operator()(uint32_t i) const {
+        Sum1 r;
+        r.v[0] = in[i];
+        return r;
+    }
+};
+
+struct StoreExclusiveU32 {
+    uint32_t* out;
+    __device__ void operator()(uint32_t i, const Sum1&, const Sum1& exclusive) const {
+        out[i] = exclusive.v[0];
+    }
+};
+
+constexpr int kSumItems = 16;
+
+// Exclusive sum of count words, in place allowed.
+inline void exclusiveSum(hipStream_t stream, const uint32_t* in, uint32_t* out, uint32_t count, Sum1* aggregates) {
+    scanPipeline<Sum1, kSumItems>(stream, count, LoadU32{in}, StoreExclusiveU32{out}, aggregates);
+}
+
+// ==== radix sort =================================================================================
+
+constexpr int kRadixBits = 8;
+constexpr int kRadixBins = 1 << kRadixBits;
+constexpr int kSortThreads = 512;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kSortWaves = kSortThreads / kWave;
+
+template <class K>
+struct ArraySource {
+    const K* __restrict__ keys;
+    const uint32_t* __restrict__ values;
+
+    __device__ __forceinline__ bool operator()(uint32_t i, K& key, uint32_t& value) const {
+        key = keys[i];
+        value = values[i];
+        return true;
+    }
+};
+
+template <class K>
+__device__ __forceinline__ uint32_t digitOf(K key, int shift, uint32_t mask) {
+    return static_cast<uint32_t>(key >> shift) & mask;
+}
+
+// hist[d * tiles + tile] = number of kept items of the tile with digit d.
+template <class K, class Source>
+__global__ __launch_bounds__(kSortThreads) void radixUpsweepKernel(Source source, uint32_t count, int shift, uint32_t mask, uint32_t tiles, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t counts[kSortWaves][kRadixBins];
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = threadIdx.x % kWave;
+
+    for (uint32_t x = threadIdx.x; x < kSortWaves * kRadixBins; x += kSortThreads) {
+        (&counts[0][0])[x] = 0;
+    }
+    __syncthreads();
+
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile + wave * kSortItems * kWave + lane;
+
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        const uint64_t i = base + static_cast<uint64_t>(j) * kWave;
+        K key = 0;
+        uint32_t value = 0;
+        const bool valid = i < count && source(static_cast<uint32_t>(i), key, value);
+        const uint32_t digit = digitOf(key, shift, mask);
+        const uint64_t peers = matchDigit<kRadixBits>(digit, valid);
+        if (valid && lane == static_cast<uint32_t>(__builtin_ctzll(peers))) {
+            counts[wave][digit] += static_cast<uint32_t>(__builtin_popcountll(peers));
+        }
+    }
+    __syncthreads();
+
+    for (uint32_t d = threadIdx.x; d < kRadixBins; d += kSortThreads) {
+        uint32_t total = 0;
+        for (int w = 0; w < kSortWaves; w++) {
+            total += counts[w][d];
+        }
+        hist[static_cast<uint64_t>(d) * tiles + blockIdx.x] = total;
+    }
+}
+
+// offsets = exclusive sum of hist (digit-major): the first output index of each digit of each tile.
+// Writes beyond `limit` outputs are dropped (a source that keeps more items than planned).
+template <class K, class Source>
+__global__ __launch_bounds__(kSortThreads) void radixDownsweepKernel(Source source, uint32_t count, int shift, uint32_t mask, uint32_t tiles, const uint32_t* __restrict__ offsets, K* __restrict__ keysOut, uint32_t* __restrict__ valuesOut, uint32_t limit) {
+    __shared__ uint32_t waveCounts[kSortWaves][kRadixBins];
+    __shared__ uint32_t digitStart[kRadixBins];
+    __shared__ uint32_t globalStart[kRadixBins];
+    __shared__ uint32_t waveSums[kRadixBins / kWave];
+    __shared__ K sortedKeys[kSortTile];
+    __shared__ uint32_t sortedValues[kSortTile];
+
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = threadIdx.x % kWave;
+
+    for (uint32_t x = threadIdx.x; x < kSortWaves * kRadixBins; x += kSortThreads) {
+        (&waveCounts[0][0])[x] = 0;
+    }
+    for (uint32_t d = threadIdx.x; d < kRadixBins; d += kSortThreads) {
+        globalStart[d] = offsets[static_cast<uint64_t>(d) * tiles + blockIdx.x];
+    }
+    __syncthreads();
+
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile + wave * kSortItems * kWave + lane;
+    K keys[kSortItems];
+    uint32_t values[kSortItems];
+    uint32_t ranks[kSortItems];
+    bool valids[kSortItems];
+
+    // Rank each item among the wave's earlier items with the same digit.
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        const uint64_t i = base + static_cast<uint64_t>(j) * kWave;
+        keys[j] = 0;
+        values[j] = 0;
+        valids[j] = i < count && source(static_cast<uint32_t>(i), keys[j], values[j]);
+        const uint32_t digit = digitOf(keys[j], shift, mask);
+        const uint64_t peers = matchDigit<kRadixBits>(digit, valids[j]);
+        const uint32_t before = valids[j] ? waveCounts[wave][digit] : 0;
+        ranks[j] = before + countBelow(peers);
+        if (valids[j] && lane == static_cast<uint32_t>(__builtin_ctzll(peers))) {
+            waveCounts[wave][digit] = before + static_cast<uint32_t>(__builtin_popcountll(peers));
+        }
+    }
+    __syncthreads();
+
+    // Per digit: the waves' prefixes, the tile's count; then the digits' starts in the tile.
+    uint32_t total = 0;
+    if (threadIdx.x < kRadixBins) {
+        const uint32_t d = threadIdx.x;
+        for (int w = 0; w < kSortWaves; w++) {
+            const uint32_t c = waveCounts[w][d];
+            waveCounts[w][d] = total;
+            total += c;
+        }
+        const uint32_t inclusive = waveInclusive<OpSum>(total);
+        if (lane == kWave - 1) {
+            waveSums[wave] = inclusive;
+        }
+        digitStart[d] = inclusive - total;
+    }
+    __syncthreads();
+    if (threadIdx.x < kRadixBins) {
+        uint32_t prefix = 0;
+        for (uint32_t w = 0; w < wave; w++) {
+            prefix += waveSums[w];
+        }
+        digitStart[threadIdx.x] += prefix;
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        if (valids[j]) {
+            const uint32_t digit = digitOf(keys[j], shift, mask);
+            const uint32_t local = digitStart[digit] + waveCounts[wave][digit] + ranks[j];
+            sortedKeys[local] = keys[j];
+            sortedValues[local] = values[j];
+        }
+    }
+    __syncthreads();
+
+    uint32_t kept = 0;
+    for (int w = 0; w < kRadixBins / kWave; w++) {
+        kept += waveSums[w];
+    }
+    for (uint32_t x = threadIdx.x; x < kept; x += kSortThreads) {
+        const K key = sortedKeys[x];
+        const uint32_t digit = digitOf(key, shift, mask);
+        const uint32_t target = globalStart[digit] + (x - digitStart[digit]);
+        if (target < limit) {
+            keysOut[target] = key;
+            valuesOut[target] = sortedValues[x];
+        }
+    }
+}
+
+// Buffers and scratch of a sort over up to `capacity` items.
+template <class K>
+struct SortBuffers {
+    K* keys[2] = {};
+    uint32_t* values[2] = {};
+    int current = 0;        // which buffer holds the data
+    uint32_t* hist = nullptr;
+    Sum1* aggregates = nullptr;
+};
+
+inline size_t radixHistWords(uint64_t capacity) {
+    return static_cast<size_t>(std::max<uint32_t>(1, ceilDiv(capacity, kSortTile))) * kRadixBins;
+}
+
+inline size_t radixAggregateBytes(uint64_t capacity) {
+    return scanScratchBytes<Sum1, kSumItems>(radixHistWords(capacity));
+}
+
+// One pass on the digit at shift (width bits), reading the source, writing buffer 1 - current.
+template <class K, class Source>
+inline void radixPass(hipStream_t stream, SortBuffers<K>& b, Source source, uint32_t count, int shift, int width, uint32_t limit = 0xFFFFFFFFu) {
+    const uint32_t tiles = std::max<uint32_t>(1, ceilDiv(count, kSortTile));
+    const uint32_t mask = (1u << width) - 1;
+    const uint32_t histWords = tiles * kRadixBins;
+    radixUpsweepKernel<K, Source><<<tiles, kSortThreads, 0, stream>>>(source, count, shift, mask, tiles, b.hist);
+    exclusiveSum(stream, b.hist, b.hist, histWords, b.aggregates);
+    radixDownsweepKernel<K, Source><<<tiles, kSortThreads, 0, stream>>>(
+        source, count, shift, mask, tiles, b.hist, b.keys[1 - b.current], b.values[1 - b.current], limit == 0xFFFFFFFFu ? count : limit);
+    b.current = 1 - b.current;
+}
+
+// Stable sort of b.keys[current], b.values[current] (count items) by bits [beginBit, endBit).
+template <class K>
+inline void radixSort(hipStream_t stream, SortBuffers<K>& b, uint32_t count, int beginBit, int endBit) {
+    if (count <= 1) {
+        return;
+    }
+    for (int shift = beginBit; shift < endBit; shift += kRadixBits) {
+        const int width = std::min(kRadixBits, endBit - shift);
+        radixPass<K>(stream, b, ArraySource<K>{b.keys[b.current], b.values[b.current]}, count, shift, width);
+    }
+}
+
+}  // namespace sadcs

@@ -1,0 +1,229 @@
+/*
+ * rank.h - Phase (a): ranking the sample suffixes.
+ *
+ * Follows the reference's scheme of passes over a shrinking list of unresolved sample suffixes
+ * (src/cuda/samplerank.cuh:1-20, :314-355): every suffix carries a group id, the number of sample
+ * suffixes known to be smaller than every member of its group; a pass splits groups by a subkey;
+ * a new group's id is the old id plus its offset in the old group; groups of one are final and
+ * leave the list. Character passes run until every group shares at least V characters, then
+ * prefix-doubling rounds use the group id at p + h (Larsson-Sadakane without in-round refinement,
+ * src/cuda/samplerank.cuh:15-18) until nothing is left.
+ *
+ * What differs, and why it suits gfx950:
+ *  - One sort per pass on a composite 64-bit key (dense group ordinal, subkey) instead of a
+ *    subkey sort, a gather of group ids and a second sort. The ordinal is the group's index in
+ *    the list, so it needs only bit_width(groups - 1) bits.
+ *  - Characters are packed by a dense code: code 0 is "past the end", the bytes present get
+ *    codes 1..sigma in byte order. Shorter suffixes then sort first with no length tag, and a
+ *    64-bit key holds 21 DNA characters, 9 of 96 printable ones, or 7 arbitrary bytes.
+ *  - Marking, id assignment, rank stores, the sorted-sample output and the compaction are one
+ *    scan pipeline (four fields: last head, group start, unresolved count, unresolved groups).
+ *  - The sorted sample positions are written as suffixes resolve (mostly coalesced, since
+ *    resolved ids rise along the list). That array is class 0 of the merge input.
+ *  - Segments: a batch ranks all its texts at once; pass 0 starts with one group per text.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "cover.h"
+#include "dev_common.h"
+#include "prim.h"
+
+namespace sadcs {
+
+template <int V>
+struct DeviceCover;  // __constant__ tables, defined by the engine
+
+// Byte -> dense character code (0 = past the end).
+struct CodeTable {
+    uint8_t code[256];
+};
+
+struct RankParams {
+    const uint64_t* text;    // padded with zero words
+    uint32_t* ranks;         // compact, indexed by pack(p)
+    uint32_t* sortedSample;  // ranks -> positions, filled as suffixes resolve
+    uint32_t samples;        // valid sample suffixes (all texts)
+    uint32_t segments;
+    const Segment* segmentTable;
+};
+
+// Subkey of one pass: characters at depth `depth`, or the group id at p + h plus one.
+struct PassSpec {
+    bool doubling;
+    uint32_t offset;    // depth (characters) or h (doubling)
+    int chars;          // characters in a character subkey
+    int charBits;       // bits per character code
+    int subkeyBits;
+};
+
+template <int V>
+__device__ __forceinline__ uint32_t packPosition(uint32_t p);
+
+template <int V>
+__device__ __forceinline__ uint32_t unpackIndex(uint32_t t);
+
+// Pass 0: the i-th valid sample suffix of the batch, and its text's index.
+template <int V, class Texts>
+__device__ __forceinline__ void firstPassEntry(const RankParams& rp, uint32_t i, uint32_t& t, uint32_t& segment) {
+    if constexpr (!Texts::kBatch) {
+        t = i;
+        segment = 0;
+    } else {
+        uint32_t lo = 0, hi = rp.segments;  // last segment with rankBase <= i
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (rp.segmentTable[mid].rankBase <= i) {
+                lo = mid;
+            } else {
+                hi = mid;
+            }
+        }
+        segment = lo;
+        const Segment s = rp.segmentTable[lo];
+        t = packPosition<V>(s.start) + (i - s.rankBase);
+    }
+}
+
+// Keys of one pass: key = (ordinal << subkeyBits) | subkey, value = compact index t.
+// kWideCodes: all 256 byte values occur, the code is the byte plus one (9 bits).
+template <int V, class Texts, bool kWideCodes>
+__global__ __launch_bounds__(256) void makePassKeysKernel(RankParams rp, Texts texts, PassSpec spec, CodeTable codes, bool first, const uint32_t* __restrict__ listT, const uint32_t* __restrict__ listOrdinal, uint32_t u, uint64_t* __restrict__ keysOut, uint32_t* __restrict__ valuesOut) {
+    __shared__ uint8_t code[256];
+    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) {
+        code[x] = codes.code[x];
+    }
+    __syncthreads();
+
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= u) {
+        return;
+    }
+
+    uint32_t t, ordinal;
+    if (first) {
+        firstPassEntry<V, Texts>(rp, i, t, ordinal);
+    } else {
+        t = listT[i];
+        ordinal = listOrdinal[i];
+    }
+    const uint32_t p = unpackIndex<V>(t);
+    const uint32_t end = texts.endOf(p);
+    uint64_t subkey = 0;
+
+    if (spec.doubling) {
+        const uint32_t q = p + spec.offset;
+        subkey = q < end ? uint64_t(rp.ranks[packPosition<V>(q)]) + 1 : 0;
+    } else {
+        const uint32_t q = p + spec.offset;
+        const uint32_t remaining = q < end ? end - q : 0;
+        const int chars = spec.chars;
+        uint64_t w = 0;
+        for (int c = 0; c < chars; c++) {
+            if ((c & 7) == 0) {
+                w = static_cast<uint32_t>(c) < remaining ? load8(rp.text, q + c) : 0;
+            }
+            const uint32_t byte = static_cast<uint32_t>(w >> 56);
+            w <<= 8;
+            const uint32_t ch = static_cast<uint32_t>(c) < remaining ? (kWideCodes ? byte + 1 : code[byte]) : 0;
+            subkey = (subkey << spec.charBits) | ch;
+        }
+    }
+
+    keysOut[i] = (spec.subkeyBits < 64 ? (uint64_t(ordinal) << spec.subkeyBits) : 0) | subkey;
+    valuesOut[i] = t;
+}
+
+// Four fields: last head (max), group start (max), unresolved before (sum), unresolved groups before (sum).
+using MarkVec = Vec<4, 0x3u>;
+constexpr int kMarkItems = 8;
+
+struct MarkState {
+    const uint64_t* keys;      // sorted composite keys
+    const uint32_t* values;    // compact indices in the same order
+    uint32_t u;
+    int subkeyBits;
+    const uint32_t* groupIds;  // ordinal -> group id
+    // outputs
+    uint32_t* ranks;
+    uint32_t* sortedSample;
+    uint32_t* nextT;
+    uint32_t* nextOrdinal;
+    uint32_t* nextGroupIds;
+    uint32_t* counts;          // [0] unresolved left, [1] groups left
+    uint32_t samples;          // entries of sortedSample
+};
+
+struct MarkFlags {
+    bool head, groupStart, resolved;
+};
+
+__device__ __forceinline__ MarkFlags markFlags(const MarkState& s, uint32_t i) {
+    const uint64_t key = s.keys[i];
+    MarkFlags f;
+    if (i == 0) {
+        f.head = true;
+        f.groupStart = true;
+    } else {
+        const uint64_t previous = s.keys[i - 1];
+        f.head = previous != key;
+        f.groupStart = s.subkeyBits >= 64 ? false : (previous >> s.subkeyBits) != (key >> s.subkeyBits);
+    }
+    const bool nextHead = i + 1 == s.u || s.keys[i + 1] != key;
+    f.resolved = f.head && nextHead;
+    return f;
+}
+
+struct MarkProducer {
+    MarkState s;
+    __device__ MarkVec operator()(uint32_t i) const {
+        const MarkFlags f = markFlags(s, i);
+        MarkVec r;
+        r.v[0] = f.head ? i : 0;
+        r.v[1] = f.groupStart ? i : 0;
+        r.v[2] = f.resolved ? 0 : 1;
+        r.v[3] = (!f.resolved && f.head) ? 1 : 0;
+        return r;
+    }
+};
+
+template <int V>
+struct MarkConsumer {
+    MarkState s;
+    __device__ void operator()(uint32_t i, const MarkVec& item, const MarkVec& exclusive) const {
+        const uint32_t lastHead = max(exclusive.v[0], item.v[0]);
+        const uint32_t groupStart = max(exclusive.v[1], item.v[1]);
+        const uint64_t key = s.keys[i];
+        const uint32_t ordinal = s.subkeyBits >= 64 ? 0 : static_cast<uint32_t>(key >> s.subkeyBits);
+        const uint32_t id = s.groupIds[ordinal] + (lastHead - groupStart);
+        const uint32_t t = s.values[i];
+
+        s.ranks[t] = id;
+        if (item.v[2] == 0) {
+            if (id < s.samples) {
+                s.sortedSample[id] = unpackIndex<V>(t);
+            }
+        } else {
+            const uint32_t slot = exclusive.v[2];
+            s.nextT[slot] = t;
+            s.nextOrdinal[slot] = exclusive.v[3] + item.v[3] - 1;
+            if (item.v[3]) {
+                s.nextGroupIds[exclusive.v[3]] = id;
+            }
+        }
+        if (i + 1 == s.u) {
+            s.counts[0] = exclusive.v[2] + item.v[2];
+            s.counts[1] = exclusive.v[3] + item.v[3];
+        }
+    }
+};
+
+__global__ void initGroupIdsKernel(const Segment* __restrict__ segments, uint32_t count, uint32_t* __restrict__ groupIds) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < count) {
+        groupIds[s] = segments[s].rankBase;
+    }
+}
+
+}  // namespace sadcs
