@@ -70,7 +70,6 @@ def main():
         return 2
 
     import numpy as np
-    import torch
 
     import sadcs_amd
     from sadcs_amd import workloads
@@ -81,30 +80,30 @@ def main():
     n = size
     print(f"# generated {description}: {n} bytes in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
 
-    device = torch.device("cuda:0")
-    d_text = torch.from_numpy(text).to(device)
-    d_sa = torch.empty(n, dtype=torch.int32, device=device)
-    d_bwt = torch.empty(n, dtype=torch.uint8, device=device)
-    torch.cuda.synchronize()
+    # Device buffers come from the library's own HIP runtime (one runtime in the process).
+    d_text = sadcs_amd.DeviceBuffer.from_host(text)
+    d_sa = sadcs_amd.DeviceBuffer(4 * n)
+    d_bwt = sadcs_amd.DeviceBuffer(n)
+    sadcs_amd.gpu_synchronize()
 
     ctx = sadcs_amd.GpuContext(n, cover=args.cover)
     stats = sadcs_amd.GpuStats()
     for w in range(args.warmup):
-        sadcs_amd.gpu_build_device(ctx, d_text.data_ptr(), n, d_sa.data_ptr(), d_bwt.data_ptr(), stats)
+        sadcs_amd.gpu_build_device(ctx, d_text.ptr, n, d_sa.ptr, d_bwt.ptr, stats)
         print(f"# warmup {w}: device {stats.device_seconds * 1e3:.1f} ms (rank {stats.ranking_seconds * 1e3:.1f}, "
               f"class {stats.class_sort_seconds * 1e3:.1f}, merge {stats.merge_seconds * 1e3:.1f}; "
               f"{stats.character_passes} character passes, {stats.doubling_rounds} doubling rounds)", file=sys.stderr, flush=True)
 
-    torch.cuda.synchronize()
+    sadcs_amd.gpu_synchronize()
     merge_kernel = []
     phases = []
     primary = -1
     start = time.perf_counter()
     for _ in range(args.steps):
-        primary = sadcs_amd.gpu_build_device(ctx, d_text.data_ptr(), n, d_sa.data_ptr(), d_bwt.data_ptr(), stats)
+        primary = sadcs_amd.gpu_build_device(ctx, d_text.ptr, n, d_sa.ptr, d_bwt.ptr, stats)
         merge_kernel.append(stats.merge_kernel_seconds)
         phases.append((stats.ranking_seconds, stats.class_sort_seconds, stats.merge_seconds))
-    torch.cuda.synchronize()
+    sadcs_amd.gpu_synchronize()
     elapsed = time.perf_counter() - start
     peak_bytes = stats.device_bytes
 
@@ -115,8 +114,8 @@ def main():
     e2e = BYTES_PER_INPUT_BYTE * n / step / HBM_PEAK
 
     # A cheap independent check of the timed output: the BWT agrees with the suffix array, row by row.
-    sa = d_sa.cpu().numpy()
-    bwt = d_bwt.cpu().numpy()
+    sa = d_sa.to_host(np.int32)
+    bwt = d_bwt.to_host(np.uint8)
     check_rows = np.random.default_rng(0).integers(0, n, 1 << 16)
     row0 = int(np.flatnonzero(sa == 0)[0])
     assert primary == row0 + 1

@@ -66,6 +66,7 @@ typedef struct sadcs_params {
     int32_t cover;               /* 32 (default) or 128 */
     uint32_t splitter_spacing;   /* G: every G-th element of a list is a splitter */
     uint32_t tile_splitters;     /* K: sorted splitters per merge tile */
+    int32_t radix_mode;          /* 0: one kernel per digit pass (look-back); 1: reduce-then-scan */
 } sadcs_params;
 
 typedef struct sadcs_ctx sadcs_ctx;
@@ -95,7 +96,7 @@ int sadcs_hip_build(sadcs_ctx* context, const uint8_t* text, int32_t n, int32_t*
                     int32_t* primary, sadcs_stats* stats);
 
 /* The same with the text already on the device and the outputs left there (no host transfers).
- * d_text: n bytes followed by at least 64 zero bytes, 8-byte aligned. d_sa (4n bytes) and d_bwt
+ * d_text: n bytes followed by at least 64 zero bytes, 16-byte aligned. d_sa (4n bytes) and d_bwt
  * (n bytes) are device buffers or null. No counterpart in the reference: the device-resident
  * form of gpuSuffixArray/gpuBwt for callers that keep texts in HBM. */
 int sadcs_hip_build_device(sadcs_ctx* context, const void* d_text, int32_t n, void* d_sa, void* d_bwt,
@@ -122,6 +123,59 @@ int sadcs_hip_batch_build(sadcs_ctx* context, const uint8_t* texts, const int64_
 
 /* Device bytes a batch of these lengths needs. */
 int64_t sadcs_hip_batch_required_bytes(const int64_t* lengths, int32_t count);
+
+/* The device layout of a batch: text i starts at offsets[i] (a multiple of 256). Returns the span;
+ * a device buffer for the batch holds span bytes plus 64 zero bytes. */
+int64_t sadcs_hip_batch_layout(const int64_t* lengths, int32_t count, int64_t* offsets);
+
+/* A batch already on the device in that layout; outputs left on the device, concatenated per text
+ * (d_sa: sum of lengths int32, local positions; d_bwt: sum of lengths bytes); primaries on the host. */
+int sadcs_hip_batch_build_device(sadcs_ctx* context, const void* d_texts, const int64_t* lengths, int32_t count,
+                                 void* d_sa, void* d_bwt, int32_t* primaries, sadcs_stats* stats);
+
+/* ---- device memory for callers without their own HIP setup (no counterpart in the reference) ----
+ * The library's HIP runtime allocates and copies, so a caller that keeps texts and outputs on the
+ * device needs no second runtime in the process. kind: 1 host to device, 2 device to host,
+ * 3 device to device. */
+void* sadcs_hip_device_alloc(size_t bytes);
+void sadcs_hip_device_free(void* pointer);
+int sadcs_hip_memcpy(void* dst, const void* src, size_t bytes, int32_t kind);
+int sadcs_hip_memset(void* dst, int32_t value, size_t bytes);
+int sadcs_hip_synchronize(void);
+
+/* ---- serving: a request loop with continuous batching (no counterpart in the reference) ----
+ * Submitted texts queue; whenever one of `workers` device streams is idle, a batch is formed from
+ * the front of the queue (at most max_batch_bytes of text and max_batch_texts texts) and admitted
+ * if its planned device memory (about 10 bytes per input byte) fits the free part of a pool of
+ * memory_budget bytes (0: 80% of the free device memory). Batches on different streams overlap.
+ * A text whose plan alone exceeds the budget completes with SADCS_NOT_ENOUGH_MEMORY. */
+typedef struct sadcs_server_params {
+    int64_t memory_budget;
+    int64_t max_batch_bytes;     /* 0: 256 MiB */
+    int32_t max_batch_texts;     /* 0: 4096 */
+    int32_t workers;             /* 0: 2 */
+    int32_t cover;               /* 0: 32 */
+    int32_t want_sa;
+    int32_t want_bwt;
+} sadcs_server_params;
+
+typedef struct sadcs_server sadcs_server;
+
+int sadcs_hip_server_create(const sadcs_server_params* params, sadcs_server** server);
+
+/* Copies the text and queues it; returns the request id (>= 0) or minus a status. */
+int64_t sadcs_hip_server_submit(sadcs_server* server, const uint8_t* text, int64_t n);
+
+/* Blocks until the request is done and copies its outputs (sa: n int32, bwt: n bytes, either may be
+ * null). latency: submission to completion; queued: submission to admission; batch_size: texts in its batch. */
+int sadcs_hip_server_wait(sadcs_server* server, int64_t id, int32_t* sa, uint8_t* bwt, int32_t* primary,
+                          double* latency_seconds, double* queued_seconds, int32_t* batch_size);
+
+/* counters[0..4]: completed requests, batches, peak pool bytes in use, budget, most batches at once. */
+void sadcs_hip_server_counters(sadcs_server* server, int64_t* counters);
+
+/* Finishes the queued requests, then stops. */
+void sadcs_hip_server_destroy(sadcs_server* server);
 
 #ifdef __cplusplus
 }

@@ -44,7 +44,13 @@ class GpuStats(ctypes.Structure):
 
 
 class GpuParams(ctypes.Structure):
-    _fields_ = [("cover", ctypes.c_int32), ("splitter_spacing", ctypes.c_uint32), ("tile_splitters", ctypes.c_uint32)]
+    _fields_ = [("cover", ctypes.c_int32), ("splitter_spacing", ctypes.c_uint32), ("tile_splitters", ctypes.c_uint32),
+                ("radix_mode", ctypes.c_int32)]
+
+
+class ServerParams(ctypes.Structure):
+    _fields_ = [("memory_budget", ctypes.c_int64), ("max_batch_bytes", ctypes.c_int64), ("max_batch_texts", ctypes.c_int32),
+                ("workers", ctypes.c_int32), ("cover", ctypes.c_int32), ("want_sa", ctypes.c_int32), ("want_bwt", ctypes.c_int32)]
 
 
 class GpuError(RuntimeError):
@@ -84,6 +90,31 @@ def _load():
     lib.sadcs_hip_batch_build.restype = ctypes.c_int
     lib.sadcs_hip_batch_required_bytes.argtypes = [ctypes.POINTER(i64), i32]
     lib.sadcs_hip_batch_required_bytes.restype = i64
+    lib.sadcs_hip_batch_layout.argtypes = [ctypes.POINTER(i64), i32, ctypes.POINTER(i64)]
+    lib.sadcs_hip_batch_layout.restype = i64
+    lib.sadcs_hip_batch_build_device.argtypes = [P, P, ctypes.POINTER(i64), i32, P, P, P, ctypes.POINTER(GpuStats)]
+    lib.sadcs_hip_batch_build_device.restype = ctypes.c_int
+    lib.sadcs_hip_server_create.argtypes = [ctypes.POINTER(ServerParams), ctypes.POINTER(P)]
+    lib.sadcs_hip_server_create.restype = ctypes.c_int
+    lib.sadcs_hip_server_submit.argtypes = [P, P, i64]
+    lib.sadcs_hip_server_submit.restype = i64
+    lib.sadcs_hip_server_wait.argtypes = [P, i64, P, P, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
+    lib.sadcs_hip_server_wait.restype = ctypes.c_int
+    lib.sadcs_hip_server_counters.argtypes = [P, ctypes.POINTER(i64)]
+    lib.sadcs_hip_server_counters.restype = None
+    lib.sadcs_hip_server_destroy.argtypes = [P]
+    lib.sadcs_hip_server_destroy.restype = None
+    lib.sadcs_hip_device_alloc.argtypes = [ctypes.c_size_t]
+    lib.sadcs_hip_device_alloc.restype = P
+    lib.sadcs_hip_device_free.argtypes = [P]
+    lib.sadcs_hip_device_free.restype = None
+    lib.sadcs_hip_memcpy.argtypes = [P, P, ctypes.c_size_t, i32]
+    lib.sadcs_hip_memcpy.restype = ctypes.c_int
+    lib.sadcs_hip_memset.argtypes = [P, i32, ctypes.c_size_t]
+    lib.sadcs_hip_memset.restype = ctypes.c_int
+    lib.sadcs_hip_synchronize.argtypes = []
+    lib.sadcs_hip_synchronize.restype = ctypes.c_int
     return lib
 
 
@@ -94,7 +125,10 @@ EXPORTED_SYMBOLS = (
     "sadcs_hip_query", "sadcs_hip_sample_size", "sadcs_hip_required_bytes", "sadcs_hip_ctx_create",
     "sadcs_hip_ctx_destroy", "sadcs_hip_build", "sadcs_hip_build_device", "sadcs_hip_rank_sample",
     "sadcs_hip_alloc_pinned", "sadcs_hip_free_pinned", "sadcs_hip_last_error", "sadcs_hip_batch_build",
-    "sadcs_hip_batch_required_bytes",
+    "sadcs_hip_batch_required_bytes", "sadcs_hip_batch_layout", "sadcs_hip_batch_build_device",
+    "sadcs_hip_server_create", "sadcs_hip_server_submit", "sadcs_hip_server_wait", "sadcs_hip_server_counters",
+    "sadcs_hip_server_destroy", "sadcs_hip_device_alloc", "sadcs_hip_device_free", "sadcs_hip_memcpy",
+    "sadcs_hip_memset", "sadcs_hip_synchronize",
 )
 
 
@@ -136,8 +170,8 @@ def _as_bytes(text):
 class GpuContext:
     """gpusa.h:84-95: device memory for inputs up to capacity bytes, kept across calls."""
 
-    def __init__(self, capacity, cover=32, splitter_spacing=0, tile_splitters=0):
-        params = GpuParams(cover, splitter_spacing, tile_splitters)
+    def __init__(self, capacity, cover=32, splitter_spacing=0, tile_splitters=0, radix_mode=0):
+        params = GpuParams(cover, splitter_spacing, tile_splitters, radix_mode)
         handle = ctypes.c_void_p()
         _check(lib.sadcs_hip_ctx_create(int(capacity), ctypes.byref(params), ctypes.byref(handle)))
         self.handle = handle
@@ -271,3 +305,125 @@ def gpu_allocate_pinned(size):
 
 def gpu_free_pinned(pointer):
     lib.sadcs_hip_free_pinned(pointer)
+
+
+def gpu_batch_layout(lengths):
+    """(offsets, span) of a device batch: text i at offsets[i]; the buffer needs span + 64 zero bytes."""
+    lengths = np.ascontiguousarray(lengths, dtype=np.int64)
+    offsets = np.zeros(max(lengths.size, 1), dtype=np.int64)
+    span = lib.sadcs_hip_batch_layout(lengths.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), lengths.size,
+                                      offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    return offsets[:lengths.size], int(span)
+
+
+def gpu_batch_build_device(context, d_texts, lengths, d_sa=None, d_bwt=None, stats=None):
+    """A device-resident batch (texts at gpu_batch_layout offsets). Returns the primaries (numpy)."""
+    lengths = np.ascontiguousarray(lengths, dtype=np.int64)
+    primaries = np.full(max(lengths.size, 1), -1, dtype=np.int32)
+    st = stats if stats is not None else GpuStats()
+    _check(lib.sadcs_hip_batch_build_device(_handle(context), ctypes.c_void_p(d_texts),
+                                            lengths.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), lengths.size,
+                                            ctypes.c_void_p(d_sa) if d_sa else None,
+                                            ctypes.c_void_p(d_bwt) if d_bwt else None,
+                                            primaries.ctypes.data, ctypes.byref(st)))
+    return primaries[:lengths.size]
+
+
+class Server:
+    """A continuous-batching construction server (sadcs_hip_server_*): submit() queues a text and
+    returns an id; wait(id) blocks until its suffix array / BWT are ready."""
+
+    def __init__(self, memory_budget=0, workers=2, max_batch_bytes=0, max_batch_texts=0, cover=32,
+                 suffix_array=True, bwt=True):
+        params = ServerParams(memory_budget, max_batch_bytes, max_batch_texts, workers, cover, int(suffix_array), int(bwt))
+        handle = ctypes.c_void_p()
+        _check(lib.sadcs_hip_server_create(ctypes.byref(params), ctypes.byref(handle)))
+        self.handle = handle
+        self.suffix_array = suffix_array
+        self.bwt = bwt
+        self._lengths = {}
+
+    def submit(self, text):
+        t = _as_bytes(text)
+        rid = lib.sadcs_hip_server_submit(self.handle, t.ctypes.data if t.size else None, t.size)
+        if rid < 0:
+            _check(-rid)
+        self._lengths[rid] = t.size
+        return rid
+
+    def wait(self, rid):
+        """Returns (sa, bwt bytes, primary, latency seconds, queued seconds, batch size)."""
+        n = self._lengths.pop(rid)
+        sa = np.empty(max(n, 1), dtype=np.int32) if self.suffix_array else None
+        b = np.empty(max(n, 1), dtype=np.uint8) if self.bwt else None
+        primary = ctypes.c_int32(-1)
+        latency = ctypes.c_double(0)
+        queued = ctypes.c_double(0)
+        batch = ctypes.c_int32(0)
+        _check(lib.sadcs_hip_server_wait(self.handle, rid, sa.ctypes.data if sa is not None else None,
+                                         b.ctypes.data if b is not None else None, ctypes.byref(primary),
+                                         ctypes.byref(latency), ctypes.byref(queued), ctypes.byref(batch)))
+        return (sa[:n] if sa is not None else None, b[:n].tobytes() if b is not None else None, primary.value,
+                latency.value, queued.value, batch.value)
+
+    def counters(self):
+        c = (ctypes.c_int64 * 5)()
+        lib.sadcs_hip_server_counters(self.handle, c)
+        return {"completed": c[0], "batches": c[1], "peak_pool_bytes": c[2], "budget": c[3], "max_concurrent_batches": c[4]}
+
+    def close(self):
+        if self.handle:
+            lib.sadcs_hip_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class DeviceBuffer:
+    """Device memory from the library's own HIP runtime (sadcs_hip_device_alloc)."""
+
+    def __init__(self, size):
+        self.size = int(size)
+        self.ptr = lib.sadcs_hip_device_alloc(self.size)
+        if not self.ptr:
+            raise GpuError(GpuStatus.NotEnoughMemory, f"device allocation of {self.size} bytes failed")
+
+    @classmethod
+    def from_host(cls, array, extra=0, zero=True):
+        a = np.ascontiguousarray(array)
+        buf = cls(a.nbytes + extra)
+        if zero and extra:
+            _check(lib.sadcs_hip_memset(buf.ptr, 0, buf.size))
+        if a.nbytes:
+            _check(lib.sadcs_hip_memcpy(buf.ptr, a.ctypes.data, a.nbytes, 1))
+        return buf
+
+    def zero(self):
+        _check(lib.sadcs_hip_memset(self.ptr, 0, self.size))
+
+    def to_host(self, dtype=np.uint8, count=None):
+        dtype = np.dtype(dtype)
+        count = self.size // dtype.itemsize if count is None else count
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            _check(lib.sadcs_hip_memcpy(out.ctypes.data, self.ptr, out.nbytes, 2))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib.sadcs_hip_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def gpu_synchronize():
+    _check(lib.sadcs_hip_synchronize())

@@ -275,23 +275,26 @@ class Arena {
 
 struct Params {
     int cover = 32;
-    uint32_t g = 64;        // splitter spacing
-    uint32_t k = 16;        // sorted splitters per tile
+    uint32_t g = 32;        // splitter spacing
+    uint32_t k = 26;        // sorted splitters per tile
+    bool lookback = true;   // radix passes with decoupled look-back (else reduce-then-scan)
 
     // Zero fields keep the defaults of the cover: tiles of at most K*G + C*(G-1) records must fit
-    // the LDS (cover 32: 7 classes, 48-byte records; cover 128: 31 classes, 72-byte records).
+    // the LDS twice per CU (cover 32: 7 classes, 64-byte records, 1049 records; cover 128:
+    // 31 classes, 88-byte records, 721 records).
     void apply(const sadcs_params* p) {
         if (p && p->cover) {
             cover = p->cover;
         }
-        g = cover == 128 ? 32 : 64;
-        k = 16;
+        g = cover == 128 ? 16 : 32;
+        k = cover == 128 ? 16 : 26;
         if (p && p->splitter_spacing) {
             g = p->splitter_spacing;
         }
         if (p && p->tile_splitters) {
             k = p->tile_splitters;
         }
+        lookback = !(p && p->radix_mode == 1);
     }
 };
 
@@ -467,6 +470,57 @@ class Pipeline {
         m_arena.release(m_ranks);
     }
 
+    // Scratch of look-back sorts over up to `capacity` items.
+    struct LookbackScratch {
+        uint64_t* status = nullptr;
+        uint32_t* digitCounts = nullptr;
+        uint32_t* tileCounters = nullptr;
+        uint32_t* errors = nullptr;
+    };
+
+    int allocateLookback(LookbackScratch& s, uint64_t capacity) {
+        if (!m_ctx.params.lookback) {
+            return SADCS_OK;
+        }
+        SADCS_TRY(m_arena.allocate(s.status, radixStatusWords(capacity)));
+        SADCS_TRY(m_arena.allocate(s.digitCounts, kMaxDigitPasses * kRadixBins));
+        SADCS_TRY(m_arena.allocate(s.tileCounters, kMaxDigitPasses));
+        SADCS_TRY(m_arena.allocate(s.errors, 1));
+        if (!m_dry) {
+            fillWords(s.errors, 1, 0);
+        }
+        return SADCS_OK;
+    }
+
+    void releaseLookback(LookbackScratch& s) {
+        m_arena.release(s.errors);
+        m_arena.release(s.tileCounters);
+        m_arena.release(s.digitCounts);
+        m_arena.release(s.status);
+    }
+
+    template <class K>
+    static void attach(SortBuffers<K>& b, const LookbackScratch& s) {
+        b.status = s.status;
+        b.digitCounts = s.digitCounts;
+        b.tileCounters = s.tileCounters;
+        b.errors = s.errors;
+    }
+
+    // Reads the look-back error word (after a synchronisation point).
+    int checkLookback(const LookbackScratch& s, const char* phase) {
+        if (m_dry || !s.errors) {
+            return SADCS_OK;
+        }
+        SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords + 12, s.errors, sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
+        SADCS_HIP(hipStreamSynchronize(m_stream));
+        if (m_ctx.hostWords[12] != 0) {
+            setError("%s: a radix look-back did not complete (internal error, code %u)", phase, m_ctx.hostWords[12]);
+            return SADCS_FAILED;
+        }
+        return SADCS_OK;
+    }
+
     static unsigned blocksFor(uint64_t count, unsigned threads = 256) {
         return static_cast<unsigned>(std::max<uint64_t>(1, (count + threads - 1) / threads));
     }
@@ -498,7 +552,9 @@ class Pipeline {
         uint32_t* byteCounts = nullptr;
 
         int status = SADCS_OK;
+        LookbackScratch lookback;
         auto cleanup = [&]() {
+            releaseLookback(lookback);
             m_arena.release(byteCounts);
             m_arena.release(counts);
             m_arena.release(markAggregates);
@@ -521,6 +577,7 @@ class Pipeline {
         if (status == SADCS_OK) status = m_arena.allocate(markAggregates, scanScratchBytes<MarkVec, kMarkItems>(M) / sizeof(MarkVec) + 1);
         if (status == SADCS_OK) status = m_arena.allocate(counts, 4);
         if (status == SADCS_OK) status = m_arena.allocate(byteCounts, 256);
+        if (status == SADCS_OK) status = allocateLookback(lookback, M);
         if (status != SADCS_OK || m_dry || M == 0) {
             cleanup();
             return status;
@@ -563,6 +620,7 @@ class Pipeline {
         sb.values[1] = values[1];
         sb.hist = hist;
         sb.aggregates = sortAggregates;
+        attach(sb, lookback);
 
         uint32_t u = M;
         uint32_t groups = m_segments;
@@ -622,6 +680,10 @@ class Pipeline {
             SADCS_HIP(hipGetLastError());
             SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords, counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
             SADCS_HIP(hipStreamSynchronize(m_stream));
+            if (const int s = checkLookback(lookback, "sample ranking"); s != SADCS_OK) {
+                cleanup();
+                return s;
+            }
 
             u = m_ctx.hostWords[0];
             groups = m_ctx.hostWords[1];
@@ -662,7 +724,9 @@ class Pipeline {
         Sum1* aggregates = nullptr;
         const uint64_t walk = largest + m_segments + static_cast<uint64_t>(m_listLength[0]);
         int status = SADCS_OK;
+        LookbackScratch lookback;
         auto cleanup = [&]() {
+            releaseLookback(lookback);
             m_arena.release(aggregates);
             m_arena.release(hist);
             m_arena.release(tmpValues);
@@ -676,6 +740,7 @@ class Pipeline {
         if (kBatch && status == SADCS_OK) status = m_arena.allocate(tmpValues, std::max<uint64_t>(largest, 1));
         if (status == SADCS_OK) status = m_arena.allocate(hist, radixHistWords(walk));
         if (status == SADCS_OK) status = m_arena.allocate(aggregates, radixAggregateBytes(walk) / sizeof(Sum1) + 1);
+        if (status == SADCS_OK) status = allocateLookback(lookback, walk);
         if (status != SADCS_OK || m_dry) {
             cleanup();
             return status;
@@ -712,14 +777,23 @@ class Pipeline {
             sb.keys[1] = keys[next];
             sb.values[1] = m_lists[d];
             sb.current = passes % 2 == 1 ? 0 : 1;
-            radixPass<Key>(m_stream, sb, source, count, 24, 8, static_cast<uint32_t>(m_listLength[d]));
-            for (int e = 0; e < extraPasses; e++) {
-                const int shift = 32 + e * kRadixBits;
-                const int width = std::min(kRadixBits, 32 + segmentBits - shift);
-                radixPass<Key>(m_stream, sb, ArraySource<Key>{sb.keys[sb.current], sb.values[sb.current]}, static_cast<uint32_t>(m_listLength[d]), shift, width);
+            if (m_ctx.params.lookback) {
+                attach(sb, lookback);
+                radixSortLookback<Key>(m_stream, sb, source, count, 24, 32 + segmentBits, static_cast<uint32_t>(m_listLength[d]));
+            } else {
+                radixPass<Key>(m_stream, sb, source, count, 24, 8, static_cast<uint32_t>(m_listLength[d]));
+                for (int e = 0; e < extraPasses; e++) {
+                    const int shift = 32 + e * kRadixBits;
+                    const int width = std::min(kRadixBits, 32 + segmentBits - shift);
+                    radixPass<Key>(m_stream, sb, ArraySource<Key>{sb.keys[sb.current], sb.values[sb.current]}, static_cast<uint32_t>(m_listLength[d]), shift, width);
+                }
             }
             SADCS_HIP(hipGetLastError());
             prev = next;
+        }
+        if (const int s = checkLookback(lookback, "class sorts"); s != SADCS_OK) {
+            cleanup();
+            return s;
         }
         cleanup();
         return SADCS_OK;
@@ -1437,6 +1511,7 @@ int buildWith(sadcs_ctx& ctx, const Layout& layout, const BuildRequest& rq, Text
 }
 
 int build(sadcs_ctx& ctx, const Layout& layout, const BuildRequest& rq, sadcs_stats* stats) {
+    (void)hipGetLastError();  // errors of earlier, unrelated calls are not this build's
     const bool batch = !layout.blockSegment.empty();
     if (ctx.params.cover == 128) {
         return batch ? buildWith<128, TextBatch>(ctx, layout, rq, TextBatch{}, stats)
@@ -1621,8 +1696,8 @@ int sadcs_hip_build_device(sadcs_ctx* context, const void* d_text, int32_t n, vo
     if (n <= 0 || (!d_sa && !d_bwt)) {
         return SADCS_OK;
     }
-    if (!d_text || (reinterpret_cast<uintptr_t>(d_text) & 7) != 0) {
-        setError("the device text must be 8-byte aligned");
+    if (!d_text || (reinterpret_cast<uintptr_t>(d_text) & 15) != 0) {
+        setError("the device text must be 16-byte aligned");
         return SADCS_FAILED;
     }
     if (context && n > context->capacity) {
@@ -1678,6 +1753,37 @@ int sadcs_hip_rank_sample(const uint8_t* text, int32_t n, uint32_t* ranks, sadcs
     rq.hostText = text;
     rq.hostRanks = ranks;
     return buildOn(nullptr, params, layout, rq, stats);
+}
+
+void* sadcs_hip_device_alloc(size_t bytes) {
+    void* pointer = nullptr;
+    if (hipMalloc(&pointer, std::max<size_t>(bytes, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return pointer;
+}
+
+void sadcs_hip_device_free(void* pointer) {
+    if (pointer) {
+        (void)hipFree(pointer);
+    }
+}
+
+int sadcs_hip_memcpy(void* dst, const void* src, size_t bytes, int32_t kind) {
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    SADCS_HIP(hipMemcpy(dst, src, bytes, k));
+    return SADCS_OK;
+}
+
+int sadcs_hip_memset(void* dst, int32_t value, size_t bytes) {
+    SADCS_HIP(hipMemset(dst, value, bytes));
+    return SADCS_OK;
+}
+
+int sadcs_hip_synchronize(void) {
+    SADCS_HIP(hipDeviceSynchronize());
+    return SADCS_OK;
 }
 
 void* sadcs_hip_alloc_pinned(size_t bytes) {
@@ -1736,6 +1842,58 @@ int sadcs_hip_batch_build(sadcs_ctx* context, const uint8_t* texts, const int64_
     return status;
 }
 
+int64_t sadcs_hip_batch_layout(const int64_t* lengths, int32_t count, int64_t* offsets) {
+    int64_t start = 0, span = 0;
+    for (int32_t i = 0; i < count; i++) {
+        if (offsets) {
+            offsets[i] = start;
+        }
+        span = start + lengths[i];
+        start = (span + kSegmentAlign - 1) / kSegmentAlign * kSegmentAlign;
+    }
+    return span;
+}
+
+int sadcs_hip_batch_build_device(sadcs_ctx* context, const void* d_texts, const int64_t* lengths, int32_t count, void* d_sa, void* d_bwt, int32_t* primaries, sadcs_stats* stats) {
+    tLastError[0] = '\0';
+    if (stats) {
+        *stats = sadcs_stats{};
+    }
+    if (count <= 0 || (!d_sa && !d_bwt)) {
+        return SADCS_OK;
+    }
+    if (!d_texts || (reinterpret_cast<uintptr_t>(d_texts) & 15) != 0) {
+        setError("the device texts must be 16-byte aligned");
+        return SADCS_FAILED;
+    }
+    uint64_t total = 0;
+    for (int32_t i = 0; i < count; i++) {
+        if (lengths[i] < 1 || lengths[i] > INT32_MAX) {
+            setError("text %d has length %lld; batched texts need 1 to 2^31-1 bytes", i, static_cast<long long>(lengths[i]));
+            return SADCS_FAILED;
+        }
+        total += static_cast<uint64_t>(lengths[i]) + kSegmentAlign;
+    }
+    if (total >= (uint64_t(1) << 31)) {
+        setError("a batch holds less than 2 GiB of text");
+        return SADCS_NOT_ENOUGH_MEMORY;
+    }
+    Params params = context ? context->params : Params{};
+    const Layout layout = layoutFor(params, lengths, count, true);
+    BuildRequest rq;
+    rq.mode = d_sa && d_bwt ? Mode::Both : d_sa ? Mode::SA : Mode::BWT;
+    rq.deviceText = static_cast<const uint8_t*>(d_texts);
+    rq.deviceSA = static_cast<uint32_t*>(d_sa);
+    rq.deviceBWT = static_cast<uint8_t*>(d_bwt);
+    std::vector<int32_t> first(static_cast<size_t>(count), -1);
+    rq.hostPrimaries = first.data();
+    const int status = buildOn(context, params, layout, rq, stats);
+    if (status == SADCS_OK && primaries && d_bwt) {
+        std::copy(first.begin(), first.end(), primaries);
+    }
+    return status;
+}
+
 int64_t sadcs_hip_batch_required_bytes(const int64_t* lengths, int32_t count) {
     if (count <= 0) {
         return 0;
@@ -1745,6 +1903,502 @@ int64_t sadcs_hip_batch_required_bytes(const int64_t* lengths, int32_t count) {
     sadcs_ctx planner;
     const size_t plan = planFor(planner, layout, Mode::Both, true, true);
     return plan == SIZE_MAX ? -1 : static_cast<int64_t>(plan);
+}
+
+}  // extern "C"
+
+// ==== serving ====================================================================================
+//
+// A request loop with continuous batching: submitted texts wait in a FIFO; a dispatcher forms a
+// batch from the front of the queue whenever a worker (one stream each) is idle and the batch's
+// planned device memory (the dry-run plan, about 10 bytes per input byte) fits in the free part of
+// the server's HBM pool. Batches of different workers run concurrently on the device; a batch's
+// texts and sample ranks stay resident from upload to the end of its merge. Requests that arrive
+// while others run are admitted as soon as a worker and memory free up.
+
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+namespace sadcs {
+
+struct ServerRequest {
+    int64_t id = 0;
+    std::vector<uint8_t> text;
+    std::vector<int32_t> sa;
+    std::vector<uint8_t> bwt;
+    int32_t primary = -1;
+    int status = SADCS_OK;
+    std::string error;
+    std::chrono::steady_clock::time_point arrival, admitted, done;
+    bool finished = false;
+    int64_t batchId = -1;
+    int32_t batchSize = 0;
+};
+
+class Server {
+ public:
+    Server(const sadcs_server_params& p) {
+        m_params.cover = p.cover ? p.cover : 32;
+        Params tmp;
+        sadcs_params sp{};
+        sp.cover = m_params.cover;
+        tmp.apply(&sp);
+        m_params = tmp;
+        m_budget = static_cast<size_t>(p.memory_budget);
+        m_workerCount = p.workers > 0 ? p.workers : 2;
+        m_maxBatchBytes = p.max_batch_bytes > 0 ? static_cast<uint64_t>(p.max_batch_bytes) : (uint64_t(256) << 20);
+        m_maxBatchTexts = p.max_batch_texts > 0 ? p.max_batch_texts : 4096;
+        m_wantSA = p.want_sa != 0;
+        m_wantBWT = p.want_bwt != 0;
+    }
+
+    ~Server() {
+        stop();
+    }
+
+    int start() {
+        sadcs_device device;
+        SADCS_TRY(queryDevice(device));
+        if (m_budget == 0) {
+            m_budget = static_cast<size_t>(std::max<int64_t>(0, device.free_bytes - kMemoryReserve) / 5 * 4);
+        }
+        if (static_cast<int64_t>(m_budget) + kMemoryReserve > device.free_bytes) {
+            setError("server budget of %zu MiB plus the reserve exceeds the free device memory", m_budget >> 20);
+            return SADCS_NOT_ENOUGH_MEMORY;
+        }
+        SADCS_TRY(ensureTables());
+        SADCS_HIP(hipMalloc(&m_poolBase, m_budget));
+        m_pool.reset(static_cast<uint8_t*>(m_poolBase), m_budget, false);
+        for (int w = 0; w < m_workerCount; w++) {
+            sadcs_ctx* ctx = nullptr;
+            const int s = createContext(0, m_params, INT32_MAX, ctx);
+            if (s != SADCS_OK) {
+                return s;
+            }
+            // Worker contexts borrow their arena from the pool for each batch.
+            (void)hipFree(ctx->arenaBase);
+            ctx->arenaBase = nullptr;
+            ctx->arenaBytes = 0;
+            m_workers.push_back(std::make_unique<Worker>());
+            m_workers.back()->ctx = ctx;
+        }
+        for (int w = 0; w < m_workerCount; w++) {
+            m_workers[w]->thread = std::thread(&Server::workerLoop, this, w);
+        }
+        m_dispatcher = std::thread(&Server::dispatchLoop, this);
+        return SADCS_OK;
+    }
+
+    void stop() {
+        {
+            std::lock_guard<std::mutex> lock(m_mutex);
+            if (m_stopped) {
+                return;
+            }
+            m_stopping = true;
+        }
+        m_wake.notify_all();
+        if (m_dispatcher.joinable()) {
+            m_dispatcher.join();
+        }
+        for (auto& w : m_workers) {
+            if (w->thread.joinable()) {
+                w->thread.join();
+            }
+            w->ctx->arenaBase = nullptr;  // never free a pool chunk as an arena
+            destroyContext(w->ctx);
+        }
+        m_workers.clear();
+        if (m_poolBase) {
+            (void)hipFree(m_poolBase);
+            m_poolBase = nullptr;
+        }
+        std::lock_guard<std::mutex> lock(m_mutex);
+        m_stopped = true;
+    }
+
+    int64_t submit(const uint8_t* text, int64_t n) {
+        auto r = std::make_shared<ServerRequest>();
+        r->text.assign(text, text + n);
+        r->arrival = std::chrono::steady_clock::now();
+        std::lock_guard<std::mutex> lock(m_mutex);
+        r->id = m_nextId++;
+        m_requests[r->id] = r;
+        m_pending.push_back(r);
+        m_wake.notify_all();
+        return r->id;
+    }
+
+    int wait(int64_t id, int32_t* sa, uint8_t* bwt, int32_t* primary, double* latency, double* queued, int32_t* batchSize) {
+        std::shared_ptr<ServerRequest> r;
+        {
+            std::unique_lock<std::mutex> lock(m_mutex);
+            auto it = m_requests.find(id);
+            if (it == m_requests.end()) {
+                setError("unknown request %lld", static_cast<long long>(id));
+                return SADCS_FAILED;
+            }
+            r = it->second;
+            m_done.wait(lock, [&] { return r->finished; });
+            m_requests.erase(id);
+        }
+        if (r->status != SADCS_OK) {
+            setError("%s", r->error.c_str());
+            return r->status;
+        }
+        if (sa && !r->sa.empty()) {
+            memcpy(sa, r->sa.data(), r->sa.size() * sizeof(int32_t));
+        }
+        if (bwt && !r->bwt.empty()) {
+            memcpy(bwt, r->bwt.data(), r->bwt.size());
+        }
+        if (primary) {
+            *primary = r->primary;
+        }
+        if (latency) {
+            *latency = std::chrono::duration<double>(r->done - r->arrival).count();
+        }
+        if (queued) {
+            *queued = std::chrono::duration<double>(r->admitted - r->arrival).count();
+        }
+        if (batchSize) {
+            *batchSize = r->batchSize;
+        }
+        return SADCS_OK;
+    }
+
+    void counters(int64_t* out) {
+        std::lock_guard<std::mutex> lock(m_mutex);
+        out[0] = m_completed;
+        out[1] = m_batches;
+        out[2] = static_cast<int64_t>(m_peakPool);
+        out[3] = static_cast<int64_t>(m_budget);
+        out[4] = m_maxConcurrent;
+    }
+
+ private:
+    struct Worker {
+        sadcs_ctx* ctx = nullptr;
+        std::thread thread;
+        std::vector<std::shared_ptr<ServerRequest>> batch;
+        uint8_t* chunk = nullptr;
+        size_t bytes = 0;
+        bool busy = false;
+        int64_t batchId = 0;
+    };
+
+    Mode mode() const {
+        return m_wantSA && m_wantBWT ? Mode::Both : m_wantSA ? Mode::SA : Mode::BWT;
+    }
+
+    size_t planOf(const std::vector<std::shared_ptr<ServerRequest>>& batch) {
+        std::vector<int64_t> lengths;
+        for (auto& r : batch) {
+            lengths.push_back(static_cast<int64_t>(r->text.size()));
+        }
+        const Layout layout = layoutFor(m_params, lengths.data(), static_cast<int32_t>(lengths.size()), batch.size() > 1);
+        sadcs_ctx planner;
+        planner.params = m_params;
+        return planFor(planner, layout, mode(), true, true);
+    }
+
+    void finish(const std::shared_ptr<ServerRequest>& r, int status, const char* error) {
+        r->status = status;
+        r->error = error ? error : "";
+        r->done = std::chrono::steady_clock::now();
+        r->finished = true;
+        m_completed++;
+    }
+
+    // With the lock held: the batch to run next, or empty.
+    // A batch from the front of the queue with its pool chunk, or an empty batch (nothing admissible
+    // now). tooLarge is set when the front request alone can never fit the budget.
+    std::vector<std::shared_ptr<ServerRequest>> formBatch(uint8_t*& chunk, size_t& bytes, bool& tooLarge) {
+        std::vector<std::shared_ptr<ServerRequest>> batch;
+        tooLarge = false;
+        uint64_t total = 0;
+        for (auto& r : m_pending) {
+            if (r->text.size() <= 1) {
+                continue;  // answered by the dispatcher directly
+            }
+            if (!batch.empty() && (total + r->text.size() > m_maxBatchBytes || static_cast<int>(batch.size()) >= m_maxBatchTexts)) {
+                break;
+            }
+            batch.push_back(r);
+            total += r->text.size() + kSegmentAlign;
+        }
+        while (!batch.empty()) {
+            const size_t plan = planOf(batch);
+            if (plan == SIZE_MAX) {
+                batch.clear();
+                break;
+            }
+            if (plan > m_budget && batch.size() == 1) {
+                tooLarge = true;  // can never fit: refused by the dispatcher
+                batch.clear();
+                break;
+            }
+            uint8_t* p = nullptr;
+            if (plan <= m_budget && m_pool.allocate(p, plan) == SADCS_OK) {
+                chunk = p;
+                bytes = plan;
+                return batch;
+            }
+            if (batch.size() == 1) {
+                batch.clear();  // waits for memory
+                break;
+            }
+            batch.resize(batch.size() / 2);  // admit fewer
+        }
+        return batch;
+    }
+
+    void dispatchLoop() {
+        std::unique_lock<std::mutex> lock(m_mutex);
+        while (true) {
+            // Texts of length 0 or 1 need no device work.
+            for (auto it = m_pending.begin(); it != m_pending.end();) {
+                auto& r = *it;
+                if (r->text.size() <= 1) {
+                    r->admitted = std::chrono::steady_clock::now();
+                    if (r->text.size() == 1) {
+                        r->sa.assign(m_wantSA ? 1 : 0, 0);
+                        r->bwt.assign(m_wantBWT ? 1 : 0, r->text[0]);
+                        r->primary = 1;
+                    } else {
+                        r->primary = 0;
+                    }
+                    finish(r, SADCS_OK, nullptr);
+                    it = m_pending.erase(it);
+                    m_done.notify_all();
+                } else {
+                    ++it;
+                }
+            }
+            if (m_stopping && m_pending.empty()) {
+                break;
+            }
+            int idle = -1;
+            for (int w = 0; w < m_workerCount; w++) {
+                if (!m_workers[w]->busy) {
+                    idle = w;
+                    break;
+                }
+            }
+            if (idle >= 0 && !m_pending.empty()) {
+                uint8_t* chunk = nullptr;
+                size_t bytes = 0;
+                bool tooLarge = false;
+                auto batch = formBatch(chunk, bytes, tooLarge);
+                if (!batch.empty()) {
+                    for (auto& r : batch) {
+                        r->admitted = std::chrono::steady_clock::now();
+                        r->batchId = m_batches;
+                        r->batchSize = static_cast<int32_t>(batch.size());
+                    }
+                    m_pending.erase(m_pending.begin(), m_pending.begin() + static_cast<std::ptrdiff_t>(batch.size()));
+                    Worker& w = *m_workers[idle];
+                    w.batch = std::move(batch);
+                    w.chunk = chunk;
+                    w.bytes = bytes;
+                    w.busy = true;
+                    w.batchId = m_batches++;
+                    m_poolInUse += bytes;
+                    m_peakPool = std::max(m_peakPool, m_poolInUse);
+                    int busy = 0;
+                    for (auto& x : m_workers) {
+                        busy += x->busy ? 1 : 0;
+                    }
+                    m_maxConcurrent = std::max<int64_t>(m_maxConcurrent, busy);
+                    m_wake.notify_all();
+                    continue;
+                }
+                // Nothing admissible: the head request never fits, or memory is held by running batches.
+                bool anyBusy = false;
+                for (auto& x : m_workers) {
+                    anyBusy = anyBusy || x->busy;
+                }
+                if (tooLarge || !anyBusy) {
+                    auto r = m_pending.front();
+                    m_pending.pop_front();
+                    r->admitted = std::chrono::steady_clock::now();
+                    finish(r, SADCS_NOT_ENOUGH_MEMORY, "the text's planned device memory exceeds the server's budget");
+                    m_done.notify_all();
+                    continue;
+                }
+            }
+            m_wake.wait(lock);
+        }
+        for (auto& w : m_workers) {
+            w->busy = w->busy;  // workers exit once idle and stopping
+        }
+        m_shutdown = true;
+        m_wake.notify_all();
+    }
+
+    void workerLoop(int index) {
+        Worker& w = *m_workers[index];
+        std::unique_lock<std::mutex> lock(m_mutex);
+        while (true) {
+            m_wake.wait(lock, [&] { return (w.busy && !w.batch.empty()) || m_shutdown; });
+            if (!(w.busy && !w.batch.empty())) {
+                break;
+            }
+            auto batch = w.batch;
+            lock.unlock();
+            runBatch(w, batch);
+            lock.lock();
+            for (auto& r : batch) {
+                r->done = std::chrono::steady_clock::now();
+                r->finished = true;
+                m_completed++;
+            }
+            m_pool.release(w.chunk);
+            m_poolInUse -= w.bytes;
+            w.batch.clear();
+            w.busy = false;
+            m_done.notify_all();
+            m_wake.notify_all();
+        }
+    }
+
+    void runBatch(Worker& w, std::vector<std::shared_ptr<ServerRequest>>& batch) {
+        std::vector<int64_t> lengths;
+        uint64_t total = 0;
+        for (auto& r : batch) {
+            lengths.push_back(static_cast<int64_t>(r->text.size()));
+            total += r->text.size();
+        }
+        std::vector<uint8_t> joined;
+        joined.reserve(total);
+        for (auto& r : batch) {
+            joined.insert(joined.end(), r->text.begin(), r->text.end());
+        }
+        const Layout layout = layoutFor(m_params, lengths.data(), static_cast<int32_t>(lengths.size()), batch.size() > 1);
+        std::vector<int32_t> sa(m_wantSA ? total : 0);
+        std::vector<uint8_t> bwt(m_wantBWT ? total : 0);
+        std::vector<int32_t> primaries(batch.size(), -1);
+        BuildRequest rq;
+        rq.mode = mode();
+        rq.hostText = joined.data();
+        rq.hostLengths = lengths.data();
+        rq.hostSA = m_wantSA ? sa.data() : nullptr;
+        rq.hostBWT = m_wantBWT ? bwt.data() : nullptr;
+        rq.hostPrimaries = primaries.data();
+        w.ctx->arenaBase = w.chunk;
+        w.ctx->arenaBytes = w.bytes;
+        w.ctx->capacity = INT32_MAX;
+        sadcs_stats stats{};
+        const int status = build(*w.ctx, layout, rq, &stats);
+        w.ctx->arenaBase = nullptr;  // the chunk belongs to the pool
+        w.ctx->arenaBytes = 0;
+        const std::string error = status == SADCS_OK ? "" : tLastError;
+        uint64_t offset = 0;
+        for (size_t i = 0; i < batch.size(); i++) {
+            auto& r = batch[i];
+            const size_t n = r->text.size();
+            r->status = status;
+            r->error = error;
+            if (status == SADCS_OK) {
+                if (m_wantSA) {
+                    r->sa.assign(sa.begin() + static_cast<std::ptrdiff_t>(offset), sa.begin() + static_cast<std::ptrdiff_t>(offset + n));
+                }
+                if (m_wantBWT) {
+                    r->bwt.assign(bwt.begin() + static_cast<std::ptrdiff_t>(offset), bwt.begin() + static_cast<std::ptrdiff_t>(offset + n));
+                    r->primary = primaries[i];
+                }
+            }
+            offset += n;
+            std::vector<uint8_t>().swap(r->text);
+        }
+    }
+
+    Params m_params;
+    size_t m_budget = 0;
+    int m_workerCount = 2;
+    uint64_t m_maxBatchBytes = 0;
+    int m_maxBatchTexts = 4096;
+    bool m_wantSA = true, m_wantBWT = true;
+    void* m_poolBase = nullptr;
+    Arena m_pool;
+    size_t m_poolInUse = 0, m_peakPool = 0;
+    std::mutex m_mutex;
+    std::condition_variable m_wake, m_done;
+    std::deque<std::shared_ptr<ServerRequest>> m_pending;
+    std::unordered_map<int64_t, std::shared_ptr<ServerRequest>> m_requests;
+    std::vector<std::unique_ptr<Worker>> m_workers;
+    std::thread m_dispatcher;
+    int64_t m_nextId = 0, m_completed = 0, m_batches = 0, m_maxConcurrent = 0;
+    bool m_stopping = false, m_shutdown = false, m_stopped = false;
+};
+
+}  // namespace sadcs
+
+struct sadcs_server {
+    std::unique_ptr<sadcs::Server> server;
+};
+
+extern "C" {
+
+int sadcs_hip_server_create(const sadcs_server_params* params, sadcs_server** out) {
+    tLastError[0] = '\0';
+    if (!out) {
+        setError("null server pointer");
+        return SADCS_FAILED;
+    }
+    *out = nullptr;
+    sadcs_server_params p{};
+    if (params) {
+        p = *params;
+    } else {
+        p.want_sa = 1;
+        p.want_bwt = 1;
+    }
+    if (p.cover != 0 && p.cover != 32 && p.cover != 128) {
+        setError("cover must be 32 or 128, not %d", p.cover);
+        return SADCS_FAILED;
+    }
+    std::unique_ptr<sadcs_server> s(new sadcs_server);
+    s->server.reset(new Server(p));
+    const int status = s->server->start();
+    if (status != SADCS_OK) {
+        return status;
+    }
+    *out = s.release();
+    return SADCS_OK;
+}
+
+int64_t sadcs_hip_server_submit(sadcs_server* server, const uint8_t* text, int64_t n) {
+    if (!server || n < 0 || n > INT32_MAX - 1024 || (n > 0 && !text)) {
+        setError("invalid submission");
+        return -SADCS_FAILED;
+    }
+    return server->server->submit(text, n);
+}
+
+int sadcs_hip_server_wait(sadcs_server* server, int64_t id, int32_t* sa, uint8_t* bwt, int32_t* primary, double* latency_seconds,
+                          double* queued_seconds, int32_t* batch_size) {
+    if (!server) {
+        setError("null server");
+        return SADCS_FAILED;
+    }
+    return server->server->wait(id, sa, bwt, primary, latency_seconds, queued_seconds, batch_size);
+}
+
+void sadcs_hip_server_counters(sadcs_server* server, int64_t* counters) {
+    if (server && counters) {
+        server->server->counters(counters);
+    }
+}
+
+void sadcs_hip_server_destroy(sadcs_server* server) {
+    if (server) {
+        server->server->stop();
+        delete server;
+    }
 }
 
 }  // extern "C"

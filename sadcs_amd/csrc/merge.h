@@ -31,7 +31,16 @@
 namespace sadcs {
 
 constexpr int kMaxLists = 32;
-constexpr int kRecordWords = 2;
+// Characters kept in a record: 32, so with the mod-32 cover (tie offsets below 32) no comparison
+// inside a tile reads the text; with mod 128 the rest comes from global memory.
+constexpr int kRecordWords = 4;
+
+// The tie-offset and window-slot tables a comparison reads, flattened [a * V + b]: copied to LDS by
+// the tile kernel for V = 32 (2 KiB), the __constant__ tables otherwise.
+struct TieTables {
+    const uint8_t* tie;
+    const uint8_t* slot;
+};
 
 template <int V>
 __device__ __forceinline__ const CoverTables<V>& deviceCover();
@@ -69,16 +78,58 @@ struct MergeRecord<V, false> {
     uint32_t position;
 };
 
+// A record with few, wide gathers: three 16-byte loads hold T[p - 1 .. p + 31] (before = T[p - 1]),
+// and the rank window comes from 16-byte loads at the aligned start of pack(p). Divergent loads cost
+// the texture path one cycle per lane each, so the count of load instructions is what matters.
+// The text buffer and the rank array are 16-byte aligned and padded past their ends.
 template <int V, class Texts>
-__device__ __forceinline__ void loadRecord(const MergeCtx& c, const Texts& texts, uint32_t p, MergeRecord<V, Texts::kBatch>& r) {
-#pragma unroll
-    for (int w = 0; w < kRecordWords; w++) {
-        r.words[w] = load8(c.text, p + 8 * w);
+__device__ __forceinline__ void loadRecord(const MergeCtx& c, const Texts& texts, uint32_t p, MergeRecord<V, Texts::kBatch>& r, uint8_t* before = nullptr) {
+    static_assert(kRecordWords == 4, "three 16-byte loads cover 32 characters and the one before");
+    const uint32_t first = p > 0 ? p - 1 : 0;
+    const uint32_t base = first & ~15u;
+    const uint4* chunks = reinterpret_cast<const uint4*>(c.text) + (base >> 4);
+    const uint4 q0 = chunks[0], q1 = chunks[1], q2 = chunks[2];
+    uint64_t w[6] = {q0.x | (uint64_t(q0.y) << 32), q0.z | (uint64_t(q0.w) << 32),
+                     q1.x | (uint64_t(q1.y) << 32), q1.z | (uint64_t(q1.w) << 32),
+                     q2.x | (uint64_t(q2.y) << 32), q2.z | (uint64_t(q2.w) << 32)};
+    const uint32_t offset = p - base;  // 0 .. 16
+    if (before) {
+        const uint32_t b = offset - 1;  // only meaningful for p > 0
+        *before = p > 0 ? static_cast<uint8_t>(w[(b >> 3) & 1] >> ((b & 7) * 8)) : 0;
     }
-    const uint32_t base = packPosition<V>(p);
+    // Shift by whole words (0, 1 or 2), then by bytes.
+    const uint32_t wordShift = offset >> 3;
+    uint64_t v[5];
 #pragma unroll
-    for (int s = 0; s < CoverSet<V>::kSize; s++) {
-        r.ranks[s] = c.ranks[base + s];
+    for (int k = 0; k < 5; k++) {
+        const uint64_t a0 = w[k];
+        const uint64_t a1 = k + 1 < 6 ? w[k + 1] : 0;
+        const uint64_t a2 = k + 2 < 6 ? w[k + 2] : 0;
+        v[k] = wordShift == 0 ? a0 : wordShift == 1 ? a1 : a2;
+    }
+    const uint32_t s = (offset & 7) * 8;
+#pragma unroll
+    for (int k = 0; k < kRecordWords; k++) {
+        r.words[k] = __builtin_bswap64(s ? (v[k] >> s) | (v[k + 1] << (64 - s)) : v[k]);
+    }
+
+    constexpr int kSize = CoverSet<V>::kSize;
+    constexpr int kQuads = (kSize + 3 + 3) / 4;  // 16-byte loads covering kSize words from any offset
+    const uint32_t index = packPosition<V>(p);
+    const uint4* quads = reinterpret_cast<const uint4*>(c.ranks) + (index >> 2);
+    uint32_t x[4 * kQuads];
+#pragma unroll
+    for (int q = 0; q < kQuads; q++) {
+        const uint4 t = quads[q];
+        x[4 * q] = t.x;
+        x[4 * q + 1] = t.y;
+        x[4 * q + 2] = t.z;
+        x[4 * q + 3] = t.w;
+    }
+    const uint32_t o = index & 3;
+#pragma unroll
+    for (int s2 = 0; s2 < kSize; s2++) {
+        r.ranks[s2] = o == 0 ? x[s2] : o == 1 ? x[s2 + 1] : o == 2 ? x[s2 + 2] : x[s2 + 3];
     }
     r.position = p;
     if constexpr (Texts::kBatch) {
@@ -99,16 +150,15 @@ __device__ __forceinline__ uint32_t remainingOf(const MergeCtx& c, const MergeRe
 
 // Record a < record b.
 template <int V, class Texts>
-__device__ __forceinline__ bool lessRecord(const MergeCtx& c, const MergeRecord<V, Texts::kBatch>& a, const MergeRecord<V, Texts::kBatch>& b) {
+__device__ __forceinline__ bool lessRecord(const MergeCtx& c, const TieTables& t, const MergeRecord<V, Texts::kBatch>& a, const MergeRecord<V, Texts::kBatch>& b) {
     if constexpr (Texts::kBatch) {
         const uint32_t sa = a.tag >> 9, sb = b.tag >> 9;
         if (sa != sb) {
             return sa < sb;
         }
     }
-    const CoverTables<V>& cv = deviceCover<V>();
     const uint32_t i = a.position, j = b.position;
-    const uint32_t k = cv.tie[i & (V - 1)][j & (V - 1)];
+    const uint32_t k = t.tie[(i & (V - 1)) * V + (j & (V - 1))];
     const uint32_t ra = remainingOf<V, Texts>(c, a), rb = remainingOf<V, Texts>(c, b);
     const uint32_t common = min(k, min(ra, rb));
 
@@ -127,7 +177,7 @@ __device__ __forceinline__ bool lessRecord(const MergeCtx& c, const MergeRecord<
     if (common < k || ra == k || rb == k) {
         return i > j;  // one is a prefix of the other: the shorter suffix is smaller
     }
-    return a.ranks[cv.slot[i & (V - 1)][k]] < b.ranks[cv.slot[j & (V - 1)][k]];
+    return a.ranks[t.slot[(i & (V - 1)) * V + k]] < b.ranks[t.slot[(j & (V - 1)) * V + k]];
 }
 
 // Suffix i (read from global memory) < the suffix of record b.
@@ -348,13 +398,92 @@ struct TileOut {
 
 constexpr int kMergeThreads = 512;
 
+// A tile's records in LDS as a structure of arrays: a wave reading one field of 64 random records
+// then hits random banks. (As 64-byte structs, the same field of different records would map to
+// 2 of the 32 banks and every comparison would serialise on bank conflicts.)
+template <int V, bool kBatch>
+struct LdsRecords {
+    static constexpr int kSize = CoverSet<V>::kSize;
+    uint32_t capacity;
+    uint64_t* words;     // [kRecordWords][capacity]
+    uint32_t* position;  // [capacity]
+    uint32_t* ranks;     // [kSize][capacity]
+    uint32_t* tag;       // [capacity], batch only
+
+    static constexpr size_t bytesPerRecord() {
+        return 8 * kRecordWords + 4 + 4 * kSize + (kBatch ? 4 : 0);
+    }
+
+    __device__ __forceinline__ void carve(unsigned char* base, uint32_t cap) {
+        capacity = cap;
+        words = reinterpret_cast<uint64_t*>(base);
+        position = reinterpret_cast<uint32_t*>(words + static_cast<size_t>(kRecordWords) * cap);
+        ranks = position + cap;
+        tag = ranks + static_cast<size_t>(kSize) * cap;
+    }
+
+    __device__ __forceinline__ void store(uint32_t e, const MergeRecord<V, kBatch>& r) {
+#pragma unroll
+        for (int w = 0; w < kRecordWords; w++) {
+            words[w * capacity + e] = r.words[w];
+        }
+        position[e] = r.position;
+#pragma unroll
+        for (int s = 0; s < kSize; s++) {
+            ranks[s * capacity + e] = r.ranks[s];
+        }
+        if constexpr (kBatch) {
+            tag[e] = r.tag;
+        }
+    }
+};
+
 template <int V, bool kBatch>
 inline size_t mergeSharedBytes(uint32_t capacity) {
-    return static_cast<size_t>(capacity) * (sizeof(MergeRecord<V, kBatch>) + 2 * sizeof(uint16_t) + 1) + 16;
+    return static_cast<size_t>(capacity) * (LdsRecords<V, kBatch>::bytesPerRecord() + 2 * sizeof(uint16_t) + 1) + 16;
+}
+
+// Record x < record y, both in LDS.
+template <int V, class Texts>
+__device__ __forceinline__ bool lessLds(const MergeCtx& c, const TieTables& t, const LdsRecords<V, Texts::kBatch>& R, uint32_t x, uint32_t y) {
+    uint32_t ra, rb;
+    if constexpr (Texts::kBatch) {
+        const uint32_t ta = R.tag[x], tb = R.tag[y];
+        if ((ta >> 9) != (tb >> 9)) {
+            return (ta >> 9) < (tb >> 9);
+        }
+        ra = ta & 0x1FF;
+        rb = tb & 0x1FF;
+    }
+    const uint32_t i = R.position[x], j = R.position[y];
+    if constexpr (!Texts::kBatch) {
+        ra = min(c.n - i, static_cast<uint32_t>(V));
+        rb = min(c.n - j, static_cast<uint32_t>(V));
+    }
+    const uint32_t k = t.tie[(i & (V - 1)) * V + (j & (V - 1))];
+    const uint32_t common = min(k, min(ra, rb));
+    const uint32_t cap = R.capacity;
+
+#pragma unroll
+    for (uint32_t w = 0; w < V / 8; w++) {
+        if (8 * w >= common) {
+            break;
+        }
+        const uint64_t mask = prefixMask(common - 8 * w);
+        const uint64_t a = (w < kRecordWords ? R.words[w * cap + x] : load8(c.text, i + 8 * w)) & mask;
+        const uint64_t b = (w < kRecordWords ? R.words[w * cap + y] : load8(c.text, j + 8 * w)) & mask;
+        if (a != b) {
+            return a < b;
+        }
+    }
+    if (common < k || ra == k || rb == k) {
+        return i > j;  // one is a prefix of the other: the shorter suffix is smaller
+    }
+    return R.ranks[t.slot[(i & (V - 1)) * V + k] * cap + x] < R.ranks[t.slot[(j & (V - 1)) * V + k] * cap + y];
 }
 
 template <int V, class Texts>
-__device__ __forceinline__ void mergeChunk(const MergeCtx& c, const MergeRecord<V, Texts::kBatch>* recs, const uint16_t* src, uint16_t* dst, const uint32_t* bounds, uint32_t runs, uint32_t o, uint32_t o1) {
+__device__ __forceinline__ void mergeChunk(const MergeCtx& c, const TieTables& t, const LdsRecords<V, Texts::kBatch>& recs, const uint16_t* src, uint16_t* dst, const uint32_t* bounds, uint32_t runs, uint32_t o, uint32_t o1) {
     const uint32_t pairs = (runs + 1) / 2;
     uint32_t j = 0;
     while (o < o1) {
@@ -370,7 +499,7 @@ __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const MergeRecord<
         uint32_t hi = min(d, lengthA);
         while (lo < hi) {
             const uint32_t mid = (lo + hi) / 2;
-            if (lessRecord<V, Texts>(c, recs[src[aStart + mid]], recs[src[aEnd + d - 1 - mid]])) {
+            if (lessLds<V, Texts>(c, t, recs, src[aStart + mid], src[aEnd + d - 1 - mid])) {
                 lo = mid + 1;
             } else {
                 hi = mid;
@@ -388,7 +517,7 @@ __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const MergeRecord<
             } else if (ia >= lengthA) {
                 takeA = false;
             } else {
-                takeA = lessRecord<V, Texts>(c, recs[src[aStart + ia]], recs[src[aEnd + ib]]);
+                takeA = lessLds<V, Texts>(c, t, recs, src[aStart + ia], src[aEnd + ib]);
             }
             dst[o++] = takeA ? src[aStart + ia++] : src[aEnd + ib++];
         }
@@ -399,17 +528,35 @@ template <int V, class Texts, int kFlags>
 __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Texts texts, ListSet lists, const uint32_t* __restrict__ rows, uint32_t capacity, TileOut out, uint32_t* __restrict__ errors) {
     using Record = MergeRecord<V, Texts::kBatch>;
     extern __shared__ __align__(16) unsigned char shared[];
-    Record* recs = reinterpret_cast<Record*>(shared);
-    uint16_t* indexA = reinterpret_cast<uint16_t*>(recs + capacity);
+    LdsRecords<V, Texts::kBatch> recs;
+    recs.carve(shared, capacity);
+    uint16_t* indexA = reinterpret_cast<uint16_t*>(shared + static_cast<size_t>(capacity) * LdsRecords<V, Texts::kBatch>::bytesPerRecord());
     uint16_t* indexB = indexA + capacity;
     uint8_t* before = reinterpret_cast<uint8_t*>(indexB + capacity);
     __shared__ uint32_t bounds[kMaxLists + 1];
     __shared__ uint32_t segStart[kMaxLists + 1];
     __shared__ uint32_t rowStart[kMaxLists];
     __shared__ uint32_t outBase;
+    constexpr bool kTablesInLds = V <= 32;
+    __shared__ uint8_t ldsTie[kTablesInLds ? V * V : 1];
+    __shared__ uint8_t ldsSlot[kTablesInLds ? V * V : 1];
 
     const int lc = lists.count;
     const uint64_t tile = blockIdx.x;
+
+    TieTables tables;
+    if constexpr (kTablesInLds) {
+        const CoverTables<V>& cv = deviceCover<V>();
+        for (uint32_t x = threadIdx.x; x < V * V; x += kMergeThreads) {
+            ldsTie[x] = (&cv.tie[0][0])[x];
+            ldsSlot[x] = (&cv.slot[0][0])[x];
+        }
+        tables.tie = ldsTie;
+        tables.slot = ldsSlot;
+    } else {
+        tables.tie = &deviceCover<V>().tie[0][0];
+        tables.slot = &deviceCover<V>().slot[0][0];
+    }
 
     __shared__ bool valid;
     if (threadIdx.x == 0) {
@@ -427,23 +574,44 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
     }
     const uint32_t size = segStart[lc];
 
-    // Load the records; the BWT character before each.
+    // Load the records, two per thread at a time so their gathers overlap; the BWT character before each.
     {
         int l = 0;
-        for (uint32_t e = threadIdx.x; e < size; e += kMergeThreads) {
-            while (l + 1 < lc && e >= segStart[l + 1]) {
-                l++;
+        for (uint32_t e0 = threadIdx.x; e0 < size; e0 += 2 * kMergeThreads) {
+            Record r[2];
+            uint32_t pos[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const uint32_t e = e0 + h * kMergeThreads;
+                pos[h] = 0;
+                if (e < size) {
+                    while (l + 1 < lc && e >= segStart[l + 1]) {
+                        l++;
+                    }
+                    uint32_t p = (kFlags & kInPlace) ? out.sa[outBase + e] : lists.ptr[l][rowStart[l] + (e - segStart[l])];
+                    if (p >= c.n) {
+                        atomicOr(errors, 2u);
+                        p = 0;
+                    }
+                    pos[h] = p;
+                }
             }
-            uint32_t p = (kFlags & kInPlace) ? out.sa[outBase + e] : lists.ptr[l][rowStart[l] + (e - segStart[l])];
-            if (p >= c.n) {
-                atomicOr(errors, 2u);
-                p = 0;
+            uint8_t characterBefore[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                loadRecord<V, Texts>(c, texts, pos[h], r[h], characterBefore + h);
             }
-            loadRecord<V, Texts>(c, texts, p, recs[e]);
-            if constexpr ((kFlags & kOutBWT) != 0) {
-                before[e] = p > 0 ? reinterpret_cast<const uint8_t*>(c.text)[p - 1] : 0;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const uint32_t e = e0 + h * kMergeThreads;
+                if (e < size) {
+                    recs.store(e, r[h]);
+                    if constexpr ((kFlags & kOutBWT) != 0) {
+                        before[e] = characterBefore[h];
+                    }
+                    indexA[e] = static_cast<uint16_t>(e);
+                }
             }
-            indexA[e] = static_cast<uint16_t>(e);
         }
     }
     __syncthreads();
@@ -455,7 +623,7 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
 
     while (runs > 1) {
         for (uint64_t o = static_cast<uint64_t>(threadIdx.x) * chunk; o < size; o += static_cast<uint64_t>(kMergeThreads) * chunk) {
-            mergeChunk<V, Texts>(c, recs, src, dst, bounds, runs, static_cast<uint32_t>(o), static_cast<uint32_t>(min(o + chunk, static_cast<uint64_t>(size))));
+            mergeChunk<V, Texts>(c, tables, recs, src, dst, bounds, runs, static_cast<uint32_t>(o), static_cast<uint32_t>(min(o + chunk, static_cast<uint64_t>(size))));
         }
         __syncthreads();
         const uint32_t pairs = (runs + 1) / 2;
@@ -474,11 +642,11 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
 
     for (uint32_t r = threadIdx.x; r < size; r += kMergeThreads) {
         const uint32_t e = src[r];
-        const uint32_t p = recs[e].position;
+        const uint32_t p = recs.position[e];
         const uint32_t row = outBase + r;
         uint32_t start = 0, rowBase = 0, segment = 0;
         if constexpr (Texts::kBatch) {
-            segment = recs[e].tag >> 9;
+            segment = recs.tag[e] >> 9;
             start = c.segments[segment].start;
             rowBase = c.segments[segment].rowBase;
         }

@@ -191,3 +191,77 @@ def test_determinism(gpu):
     a = gpu.gpu_build(text)
     b = gpu.gpu_build(text)
     assert np.array_equal(a[0], b[0]) and a[1:] == b[1:]
+
+
+def test_serving_results_equal_alone_and_in_mix(gpu):
+    """Requests submitted together are batched and overlapped; each result must equal the same text
+    built alone (and the oracle)."""
+    from sadcs_amd import workloads
+    lengths, kinds = workloads.serving_requests(40, seed=21, median=20_000, sigma=1.2, cap=400_000)
+    texts = [workloads.serving_text(n, k, 100 + i).tobytes() for i, (n, k) in enumerate(zip(lengths, kinds))]
+    texts += [b"banana", b"a", b""]
+    with gpu.Server(memory_budget=1 << 30, workers=2, max_batch_bytes=1 << 20) as server:
+        ids = [server.submit(t) for t in texts]
+        results = [server.wait(i) for i in ids]
+        counters = server.counters()
+    print(f"serving counters {counters}", flush=True)
+    assert counters["batches"] < len(texts), "requests were not batched"
+    for i, text in enumerate(texts):
+        sa, b, primary, latency, queued, batch = results[i]
+        alone = gpu.gpu_build(text, True, True)
+        assert np.array_equal(sa, alone[0]), f"request {i}: SA differs from the single build"
+        assert (b, primary) == (alone[1], alone[2]), f"request {i}: BWT differs from the single build"
+        if len(text) <= 20000:
+            assert np.array_equal(sa, oracle.naive_sa(text))
+        assert latency >= queued >= 0
+
+
+def test_serving_refuses_over_budget(gpu):
+    with gpu.Server(memory_budget=64 << 20, workers=1) as server:
+        big = server.submit(oracle.Rng(2).random_text(8_000_000, 4))
+        small = server.submit(b"mississippi")
+        with pytest.raises(gpu.GpuError) as info:
+            server.wait(big)
+        assert info.value.status == gpu.GpuStatus.NotEnoughMemory
+        sa, b, primary, *_ = server.wait(small)
+        assert sa.tolist() == oracle.naive_sa(b"mississippi").tolist()
+
+
+def test_device_batch_equals_single(gpu):
+    from sadcs_amd import workloads
+    texts = [workloads.dna_like(n, 30 + i) for i, n in enumerate([1500, 70_000, 3000, 250_000])]
+    texts += [workloads.markov(n, 40 + i) for i, n in enumerate([2048, 100_000, 9999])]
+    lengths = np.array([t.size for t in texts], dtype=np.int64)
+    offsets, span = gpu.gpu_batch_layout(lengths)
+    host = np.zeros(span + 64, dtype=np.uint8)
+    for t, o in zip(texts, offsets):
+        host[o:o + t.size] = t
+    d_text = gpu.DeviceBuffer.from_host(host)
+    total = int(lengths.sum())
+    d_sa = gpu.DeviceBuffer(4 * total)
+    d_bwt = gpu.DeviceBuffer(total)
+    primaries = gpu.gpu_batch_build_device(None, d_text.ptr, lengths, d_sa.ptr, d_bwt.ptr)
+    sa = d_sa.to_host(np.int32)
+    bwt = d_bwt.to_host(np.uint8)
+    offset = 0
+    for i, t in enumerate(texts):
+        n = t.size
+        want_sa, want_bwt, want_primary = gpu.gpu_build(t, True, True)
+        assert np.array_equal(sa[offset:offset + n], want_sa), f"text {i}"
+        assert bwt[offset:offset + n].tobytes() == want_bwt and primaries[i] == want_primary, f"text {i}"
+        offset += n
+
+
+def test_device_single_equals_host(gpu):
+    from sadcs_amd import workloads
+    text = workloads.markov(3_000_000, 77)
+    d_text = gpu.DeviceBuffer.from_host(text, extra=64)
+    d_sa = gpu.DeviceBuffer(4 * text.size)
+    d_bwt = gpu.DeviceBuffer(text.size)
+    with gpu.GpuContext(text.size) as ctx:
+        stats = gpu.GpuStats()
+        primary = gpu.gpu_build_device(ctx, d_text.ptr, text.size, d_sa.ptr, d_bwt.ptr, stats)
+    want = oracle.ref_suffix_array(text.tobytes()) if oracle.ref_available() else oracle.suffix_array(text.tobytes())
+    assert np.array_equal(d_sa.to_host(np.int32), want)
+    assert (d_bwt.to_host(np.uint8).tobytes(), primary) == oracle.bwt_from_sa(text.tobytes(), want)
+    assert stats.merge_kernel_seconds > 0 and stats.merge_kernel_elements == text.size
