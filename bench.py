@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--cover", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-mib", type=int, default=256)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first copy BYTES device to device three times (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -85,6 +87,17 @@ def main():
     d_sa = sadcs_amd.DeviceBuffer(4 * n)
     d_bwt = sadcs_amd.DeviceBuffer(n)
     sadcs_amd.gpu_synchronize()
+
+    if args.copy_calibration:
+        source = sadcs_amd.DeviceBuffer(args.copy_calibration)
+        target = sadcs_amd.DeviceBuffer(args.copy_calibration)
+        source.zero()
+        for _ in range(3):
+            sadcs_amd.gpu_copy_device(target.ptr, source.ptr, args.copy_calibration)
+        sadcs_amd.gpu_synchronize()
+        source.free()
+        target.free()
+        print(f"# copied {args.copy_calibration} bytes device to device 3 times", file=sys.stderr, flush=True)
 
     ctx = sadcs_amd.GpuContext(n, cover=args.cover)
     stats = sadcs_amd.GpuStats()

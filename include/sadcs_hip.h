@@ -66,7 +66,6 @@ typedef struct sadcs_params {
     int32_t cover;               /* 32 (default) or 128 */
     uint32_t splitter_spacing;   /* G: every G-th element of a list is a splitter */
     uint32_t tile_splitters;     /* K: sorted splitters per merge tile */
-    int32_t radix_mode;          /* 0: one kernel per digit pass (look-back); 1: reduce-then-scan */
 } sadcs_params;
 
 typedef struct sadcs_ctx sadcs_ctx;

@@ -44,8 +44,7 @@ class GpuStats(ctypes.Structure):
 
 
 class GpuParams(ctypes.Structure):
-    _fields_ = [("cover", ctypes.c_int32), ("splitter_spacing", ctypes.c_uint32), ("tile_splitters", ctypes.c_uint32),
-                ("radix_mode", ctypes.c_int32)]
+    _fields_ = [("cover", ctypes.c_int32), ("splitter_spacing", ctypes.c_uint32), ("tile_splitters", ctypes.c_uint32)]
 
 
 class ServerParams(ctypes.Structure):
@@ -170,8 +169,8 @@ def _as_bytes(text):
 class GpuContext:
     """gpusa.h:84-95: device memory for inputs up to capacity bytes, kept across calls."""
 
-    def __init__(self, capacity, cover=32, splitter_spacing=0, tile_splitters=0, radix_mode=0):
-        params = GpuParams(cover, splitter_spacing, tile_splitters, radix_mode)
+    def __init__(self, capacity, cover=32, splitter_spacing=0, tile_splitters=0):
+        params = GpuParams(cover, splitter_spacing, tile_splitters)
         handle = ctypes.c_void_p()
         _check(lib.sadcs_hip_ctx_create(int(capacity), ctypes.byref(params), ctypes.byref(handle)))
         self.handle = handle
@@ -427,3 +426,7 @@ class DeviceBuffer:
 
 def gpu_synchronize():
     _check(lib.sadcs_hip_synchronize())
+
+
+def gpu_copy_device(dst, src, size):
+    _check(lib.sadcs_hip_memcpy(dst, src, int(size), 3))

@@ -119,6 +119,16 @@ __global__ void firstBwtBytesKernel(const uint8_t* __restrict__ bytes, const Seg
     }
 }
 
+// The same from the lines (T[x] is byte x mod 32 + 1 of line x / 32), for a BWT that overwrites
+// the text.
+__global__ void firstBwtBytesFromLinesKernel(const uint4* __restrict__ lines, const Segment* __restrict__ segments, uint32_t count, uint8_t* __restrict__ bwt) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < count) {
+        const uint64_t x = segments[s].end - 1;
+        bwt[segments[s].rowBase] = reinterpret_cast<const uint8_t*>(lines)[(x / kLinePositions) * kLineBytes + x % kLinePositions + 1];
+    }
+}
+
 // ==== errors =====================================================================================
 
 thread_local char tLastError[512] = "";
@@ -275,26 +285,19 @@ class Arena {
 
 struct Params {
     int cover = 32;
-    uint32_t g = 32;        // splitter spacing
-    uint32_t k = 26;        // sorted splitters per tile
-    bool lookback = true;   // radix passes with decoupled look-back (else reduce-then-scan)
+    uint32_t g = 16;        // splitter spacing
+    uint32_t k = 0;         // sorted splitters per tile; 0: the most that keep two tiles per CU
 
-    // Zero fields keep the defaults of the cover: tiles of at most K*G + C*(G-1) records must fit
-    // the LDS twice per CU (cover 32: 7 classes, 64-byte records, 1049 records; cover 128:
-    // 31 classes, 88-byte records, 721 records).
     void apply(const sadcs_params* p) {
         if (p && p->cover) {
             cover = p->cover;
         }
-        g = cover == 128 ? 16 : 32;
-        k = cover == 128 ? 16 : 26;
         if (p && p->splitter_spacing) {
             g = p->splitter_spacing;
         }
         if (p && p->tile_splitters) {
             k = p->tile_splitters;
         }
-        lookback = !(p && p->radix_mode == 1);
     }
 };
 
@@ -312,6 +315,7 @@ struct Job {
     bool wantSA = false;
     bool wantBWT = false;
     uint32_t* firstRows = nullptr;          // device, one per text (BWT)
+    bool textScratch = false;               // the text is this call's upload: free to overwrite late in the merge
     uint32_t maxLength = 0;
     uint64_t totalLength = 0;               // positions in texts (output rows)
 };
@@ -403,6 +407,8 @@ class Pipeline {
         m_ctxMerge.text = reinterpret_cast<const uint64_t*>(job.text);
         m_ctxMerge.segments = job.deviceSegments;
         m_ctxMerge.n = job.span;
+        m_g = ctx.params.g;
+        m_k = ctx.params.k ? ctx.params.k : defaultTileSplitters<V, kBatch>(m_g);
     }
 
     ~Pipeline() {
@@ -451,6 +457,7 @@ class Pipeline {
     uint32_t m_segments;
     uint32_t m_samples;
     MergeCtx m_ctxMerge{};
+    uint32_t m_g = 0, m_k = 0;  // splitter spacing, sorted splitters per tile
 
     uint32_t* m_ranks = nullptr;
     uint32_t* m_lists[kClasses] = {};
@@ -468,57 +475,6 @@ class Pipeline {
             m_arena.release(m_lists[d]);
         }
         m_arena.release(m_ranks);
-    }
-
-    // Scratch of look-back sorts over up to `capacity` items.
-    struct LookbackScratch {
-        uint64_t* status = nullptr;
-        uint32_t* digitCounts = nullptr;
-        uint32_t* tileCounters = nullptr;
-        uint32_t* errors = nullptr;
-    };
-
-    int allocateLookback(LookbackScratch& s, uint64_t capacity) {
-        if (!m_ctx.params.lookback) {
-            return SADCS_OK;
-        }
-        SADCS_TRY(m_arena.allocate(s.status, radixStatusWords(capacity)));
-        SADCS_TRY(m_arena.allocate(s.digitCounts, kMaxDigitPasses * kRadixBins));
-        SADCS_TRY(m_arena.allocate(s.tileCounters, kMaxDigitPasses));
-        SADCS_TRY(m_arena.allocate(s.errors, 1));
-        if (!m_dry) {
-            fillWords(s.errors, 1, 0);
-        }
-        return SADCS_OK;
-    }
-
-    void releaseLookback(LookbackScratch& s) {
-        m_arena.release(s.errors);
-        m_arena.release(s.tileCounters);
-        m_arena.release(s.digitCounts);
-        m_arena.release(s.status);
-    }
-
-    template <class K>
-    static void attach(SortBuffers<K>& b, const LookbackScratch& s) {
-        b.status = s.status;
-        b.digitCounts = s.digitCounts;
-        b.tileCounters = s.tileCounters;
-        b.errors = s.errors;
-    }
-
-    // Reads the look-back error word (after a synchronisation point).
-    int checkLookback(const LookbackScratch& s, const char* phase) {
-        if (m_dry || !s.errors) {
-            return SADCS_OK;
-        }
-        SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords + 12, s.errors, sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
-        SADCS_HIP(hipStreamSynchronize(m_stream));
-        if (m_ctx.hostWords[12] != 0) {
-            setError("%s: a radix look-back did not complete (internal error, code %u)", phase, m_ctx.hostWords[12]);
-            return SADCS_FAILED;
-        }
-        return SADCS_OK;
     }
 
     static unsigned blocksFor(uint64_t count, unsigned threads = 256) {
@@ -552,9 +508,7 @@ class Pipeline {
         uint32_t* byteCounts = nullptr;
 
         int status = SADCS_OK;
-        LookbackScratch lookback;
         auto cleanup = [&]() {
-            releaseLookback(lookback);
             m_arena.release(byteCounts);
             m_arena.release(counts);
             m_arena.release(markAggregates);
@@ -577,7 +531,6 @@ class Pipeline {
         if (status == SADCS_OK) status = m_arena.allocate(markAggregates, scanScratchBytes<MarkVec, kMarkItems>(M) / sizeof(MarkVec) + 1);
         if (status == SADCS_OK) status = m_arena.allocate(counts, 4);
         if (status == SADCS_OK) status = m_arena.allocate(byteCounts, 256);
-        if (status == SADCS_OK) status = allocateLookback(lookback, M);
         if (status != SADCS_OK || m_dry || M == 0) {
             cleanup();
             return status;
@@ -620,7 +573,6 @@ class Pipeline {
         sb.values[1] = values[1];
         sb.hist = hist;
         sb.aggregates = sortAggregates;
-        attach(sb, lookback);
 
         uint32_t u = M;
         uint32_t groups = m_segments;
@@ -680,10 +632,6 @@ class Pipeline {
             SADCS_HIP(hipGetLastError());
             SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords, counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
             SADCS_HIP(hipStreamSynchronize(m_stream));
-            if (const int s = checkLookback(lookback, "sample ranking"); s != SADCS_OK) {
-                cleanup();
-                return s;
-            }
 
             u = m_ctx.hostWords[0];
             groups = m_ctx.hostWords[1];
@@ -724,9 +672,7 @@ class Pipeline {
         Sum1* aggregates = nullptr;
         const uint64_t walk = largest + m_segments + static_cast<uint64_t>(m_listLength[0]);
         int status = SADCS_OK;
-        LookbackScratch lookback;
         auto cleanup = [&]() {
-            releaseLookback(lookback);
             m_arena.release(aggregates);
             m_arena.release(hist);
             m_arena.release(tmpValues);
@@ -740,7 +686,6 @@ class Pipeline {
         if (kBatch && status == SADCS_OK) status = m_arena.allocate(tmpValues, std::max<uint64_t>(largest, 1));
         if (status == SADCS_OK) status = m_arena.allocate(hist, radixHistWords(walk));
         if (status == SADCS_OK) status = m_arena.allocate(aggregates, radixAggregateBytes(walk) / sizeof(Sum1) + 1);
-        if (status == SADCS_OK) status = allocateLookback(lookback, walk);
         if (status != SADCS_OK || m_dry) {
             cleanup();
             return status;
@@ -777,23 +722,14 @@ class Pipeline {
             sb.keys[1] = keys[next];
             sb.values[1] = m_lists[d];
             sb.current = passes % 2 == 1 ? 0 : 1;
-            if (m_ctx.params.lookback) {
-                attach(sb, lookback);
-                radixSortLookback<Key>(m_stream, sb, source, count, 24, 32 + segmentBits, static_cast<uint32_t>(m_listLength[d]));
-            } else {
-                radixPass<Key>(m_stream, sb, source, count, 24, 8, static_cast<uint32_t>(m_listLength[d]));
-                for (int e = 0; e < extraPasses; e++) {
-                    const int shift = 32 + e * kRadixBits;
-                    const int width = std::min(kRadixBits, 32 + segmentBits - shift);
-                    radixPass<Key>(m_stream, sb, ArraySource<Key>{sb.keys[sb.current], sb.values[sb.current]}, static_cast<uint32_t>(m_listLength[d]), shift, width);
-                }
+            radixPass<Key>(m_stream, sb, source, count, 24, 8, static_cast<uint32_t>(m_listLength[d]));
+            for (int e = 0; e < extraPasses; e++) {
+                const int shift = 32 + e * kRadixBits;
+                const int width = std::min(kRadixBits, 32 + segmentBits - shift);
+                radixPass<Key>(m_stream, sb, ArraySource<Key>{sb.keys[sb.current], sb.values[sb.current]}, static_cast<uint32_t>(m_listLength[d]), shift, width);
             }
             SADCS_HIP(hipGetLastError());
             prev = next;
-        }
-        if (const int s = checkLookback(lookback, "class sorts"); s != SADCS_OK) {
-            cleanup();
-            return s;
         }
         cleanup();
         return SADCS_OK;
@@ -817,17 +753,17 @@ class Pipeline {
 
     template <int kFlags>
     int launchTiles(const ListSet& lists, const uint32_t* rows, uint32_t tiles, const TileOut& out, uint32_t* errors) {
-        const uint32_t capacity = mergeCapacity<V>(m_ctx.params.g, m_ctx.params.k);
+        const uint32_t capacity = mergeCapacity<V>(m_g, m_k);
         const size_t shared = mergeSharedBytes<V, kBatch>(capacity);
         if (m_dry) {
             return SADCS_OK;
         }
         constexpr bool kTop = (kFlags & kOutIndex) == 0;
-        auto kernel = mergeTilesKernel<V, Texts, kFlags>;
-        SADCS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shared)));
         if (kTop && m_ctx.kernelTimer[0]) {
             SADCS_HIP(hipEventRecord(m_ctx.kernelTimer[0], m_stream));
         }
+        auto kernel = mergeTilesKernel<V, Texts, kFlags>;
+        SADCS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shared)));
         kernel<<<tiles, kMergeThreads, shared, m_stream>>>(m_ctxMerge, m_texts, lists, rows, capacity, out, errors);
         SADCS_HIP(hipGetLastError());
         if (kTop && m_ctx.kernelTimer[0]) {
@@ -839,10 +775,34 @@ class Pipeline {
     }
 
     bool m_timedMerge = false;
+    uint4* m_lines = nullptr;
+    bool m_bwtInText = false;
+
+    // Cover 32: the lines of the whole span, then the ranks are no longer needed (the lines hold
+    // them). Other covers read records from the text and the rank array.
+    int buildLines() {
+        if constexpr (V != kLinePositions) {
+            return SADCS_OK;
+        } else {
+            const uint64_t count = lineCount(m_job.span);
+            SADCS_TRY(m_arena.allocate(m_lines, count * (kLineBytes / sizeof(uint4))));
+            if (!m_dry) {
+                const uint64_t threads = count * (kLineBytes / sizeof(uint4));
+                buildLinesKernel<<<static_cast<unsigned>((threads + 255) / 256), 256, 0, m_stream>>>(
+                    reinterpret_cast<const uint4*>(m_job.text), m_ranks, count, m_lines);
+                SADCS_HIP(hipGetLastError());
+            }
+            // Stream order keeps the build's reads of the ranks ahead of any reuse of their memory.
+            m_arena.release(m_ranks);
+            m_ctxMerge.ranks = nullptr;
+            m_ctxMerge.lines = m_lines;
+            return SADCS_OK;
+        }
+    }
 
     // Rows of the tiles of one level: (tiles + 1) rows of lists.count + 1 words, allocated here.
     int levelRows(const ListSet& lists, uint32_t*& rows, uint32_t& tiles, uint32_t* errors) {
-        const uint32_t g = m_ctx.params.g, k = m_ctx.params.k;
+        const uint32_t g = m_g, k = m_k;
         const uint32_t capacity = mergeCapacity<V>(g, k);
         const int lc = lists.count;
         const uint64_t total = lists.indexBase[lc];
@@ -951,8 +911,10 @@ class Pipeline {
         uint32_t* ownSA = nullptr;
         uint8_t* ownBWT = nullptr;
 
-        if (status == SADCS_OK && m_job.wantSA && m_job.wantBWT) {
-            // Tile-major copy into the SA buffer, free the lists, then the BWT in their place.
+        if (status == SADCS_OK && m_job.wantSA) {
+            // Tile-major copy of the lists into the SA buffer, free the lists; cover 32 then builds
+            // the lines in their place and frees the ranks. Every tile reads its positions from its
+            // own range of the SA buffer and writes its rows there.
             out.sa = m_job.sa;
             if (!out.sa) {
                 status = m_arena.allocate(ownSA, m_job.span);
@@ -965,25 +927,30 @@ class Pipeline {
             for (int d = 0; d < kClasses; d++) {
                 m_arena.release(m_lists[d]);
             }
-            out.bwt = m_job.bwt;
-            if (status == SADCS_OK && !out.bwt) {
-                status = m_arena.allocate(ownBWT, m_job.span);
-                out.bwt = ownBWT;
+            if (status == SADCS_OK) {
+                status = buildLines();
             }
-            if (status == SADCS_OK && !m_dry) {
-                firstBwtBytesKernel<<<blocksFor(m_segments), 256, 0, m_stream>>>(m_job.text, m_job.deviceSegments, m_segments, out.bwt);
+            if (status == SADCS_OK && m_job.wantBWT) {
+                out.bwt = m_job.bwt;
+                if (!out.bwt && m_lines && m_job.textScratch) {
+                    out.bwt = const_cast<uint8_t*>(m_job.text);  // the merge reads only the lines now
+                    m_bwtInText = true;
+                }
+                if (!out.bwt) {
+                    status = m_arena.allocate(ownBWT, m_job.span);
+                    out.bwt = ownBWT;
+                }
+            }
+            if (status == SADCS_OK && m_job.wantBWT && !m_dry) {
+                if (m_lines) {
+                    firstBwtBytesFromLinesKernel<<<blocksFor(m_segments), 256, 0, m_stream>>>(m_lines, m_job.deviceSegments, m_segments, out.bwt);
+                } else {
+                    firstBwtBytesKernel<<<blocksFor(m_segments), 256, 0, m_stream>>>(m_job.text, m_job.deviceSegments, m_segments, out.bwt);
+                }
             }
             if (status == SADCS_OK) {
-                status = launchTiles<kOutSA | kOutBWT | kInPlace>(mergeLists, rows, tiles, out, errors);
-            }
-        } else if (status == SADCS_OK && m_job.wantSA) {
-            out.sa = m_job.sa;
-            if (!out.sa) {
-                status = m_arena.allocate(ownSA, m_job.span);
-                out.sa = ownSA;
-            }
-            if (status == SADCS_OK) {
-                status = launchTiles<kOutSA>(mergeLists, rows, tiles, out, errors);
+                status = m_job.wantBWT ? launchTiles<kOutSA | kOutBWT | kInPlace>(mergeLists, rows, tiles, out, errors)
+                                       : launchTiles<kOutSA | kInPlace>(mergeLists, rows, tiles, out, errors);
             }
         } else if (status == SADCS_OK && m_job.wantBWT) {
             out.bwt = m_job.bwt;
@@ -1013,7 +980,10 @@ class Pipeline {
             }
         }
         m_outSA = ownSA;
-        m_outBWT = ownBWT;
+        m_outBWT = m_bwtInText ? const_cast<uint8_t*>(m_job.text) : ownBWT;
+        m_ownBWT = ownBWT;
+        m_ctxMerge.lines = nullptr;
+        m_arena.release(m_lines);
         m_arena.release(rows);
         if (ownFirstRows) {
             m_firstRowsOwned = firstRows;
@@ -1025,12 +995,13 @@ class Pipeline {
  public:
     // Outputs the pipeline allocated itself (host-memory calls), valid until destruction.
     uint32_t* m_outSA = nullptr;
-    uint8_t* m_outBWT = nullptr;
+    uint8_t* m_outBWT = nullptr;      // the BWT, possibly in the text's buffer
+    uint8_t* m_ownBWT = nullptr;      // the BWT's own allocation, if any
     uint32_t* m_firstRowsOwned = nullptr;
 
     void releaseOutputs() {
         m_arena.release(m_outSA);
-        m_arena.release(m_outBWT);
+        m_arena.release(m_ownBWT);
         m_arena.release(m_firstRowsOwned);
     }
 };
@@ -1148,6 +1119,7 @@ size_t planBytes(const sadcs_ctx& base, const Layout& layout, Mode mode, bool ow
     job.wantSA = mode == Mode::SA || mode == Mode::Both;
     job.wantBWT = mode == Mode::BWT || mode == Mode::Both;
     job.firstRows = in.firstRows;
+    job.textScratch = ownText;
     job.maxLength = layout.maxLength;
     job.totalLength = layout.totalLength;
     uint32_t* dummySA = ownOutputs ? nullptr : reinterpret_cast<uint32_t*>(uintptr_t(1) << 40);
@@ -1453,6 +1425,7 @@ int buildWith(sadcs_ctx& ctx, const Layout& layout, const BuildRequest& rq, Text
     job.sa = job.wantSA ? rq.deviceSA : nullptr;
     job.bwt = job.wantBWT ? rq.deviceBWT : nullptr;
     job.firstRows = in.firstRows;
+    job.textScratch = !deviceCall;
     job.maxLength = layout.maxLength;
     job.totalLength = layout.totalLength;
 
@@ -1530,11 +1503,12 @@ bool validParams(const Params& p) {
         setError("cover must be 32 or 128, not %d", p.cover);
         return false;
     }
-    if (p.g < 2 || p.k < 1 || p.g > 4096 || p.k > 4096) {
+    if (p.g < 2 || p.g > 4096 || p.k > 4096) {
         setError("invalid merge parameters (splitter spacing %u, tile splitters %u)", p.g, p.k);
         return false;
     }
-    const uint32_t capacity = p.cover == 128 ? mergeCapacity<128>(p.g, p.k) : mergeCapacity<32>(p.g, p.k);
+    const uint32_t k = p.k ? p.k : p.cover == 128 ? defaultTileSplitters<128, true>(p.g) : defaultTileSplitters<32, true>(p.g);
+    const uint32_t capacity = p.cover == 128 ? mergeCapacity<128>(p.g, k) : mergeCapacity<32>(p.g, k);
     const size_t shared = p.cover == 128 ? mergeSharedBytes<128, true>(capacity) : mergeSharedBytes<32, true>(capacity);
     if (capacity > 65535 || shared > 160 * 1024 - 1024) {
         setError("merge tiles of up to %u elements need %zu bytes of LDS, more than a workgroup has", capacity, shared);

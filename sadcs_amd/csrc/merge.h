@@ -42,6 +42,23 @@ struct TieTables {
     const uint8_t* slot;
 };
 
+// Cover 32 as a bit mask: bit k of rotr(mask, r) says whether r + k is a sample position (mod 32).
+// The tie offset of residues a and b is then the lowest set bit of rotr(mask, a) & rotr(mask, b),
+// and the slot of a + k in a's rank window (samples in [a, a + k)) a population count.
+template <int V>
+constexpr uint32_t coverMask() {
+    static_assert(V == 32, "a 32-bit mask");
+    uint32_t mask = 0;
+    for (int e = 0; e < CoverSet<V>::kSize; e++) {
+        mask |= 1u << CoverSet<V>::kElements[e];
+    }
+    return mask;
+}
+
+__device__ __forceinline__ uint32_t windowOf32(uint32_t mask, uint32_t p) {
+    return __builtin_rotateright32(mask, p & 31);
+}
+
 template <int V>
 __device__ __forceinline__ const CoverTables<V>& deviceCover();
 
@@ -52,8 +69,52 @@ struct MergeCtx {
     const uint64_t* text;
     const uint32_t* ranks;
     const Segment* segments;
-    uint32_t n;  // single text: its length
+    uint32_t n;          // single text: its length
+    const uint4* lines;  // cover 32: one line per 32 positions (buildLinesKernel), or null
 };
+
+// ---- lines: one 128-byte line per 32 text positions (cover 32) ------------------------------
+//
+// Line b holds T[32b - 1 .. 32b + 62] (bytes 0..63, T[-1] = 0) and the ranks of the 14 sample
+// positions in [32b, 32b + 64), pack indices 7b .. 7b + 13 (bytes 64..119). A record of any p in
+// [32b, 32b + 32) - T[p - 1 .. p + 31] and the 7 ranks from the first sample at or after p - lies
+// inside line b, so a record costs one line of HBM traffic instead of the 2 to 3 lines its text
+// window and rank window straddle when read from the text and the rank array.
+constexpr int kLineBytes = 128;
+constexpr int kLinePositions = 32;
+
+inline uint64_t lineCount(uint64_t span) {
+    return (span + kLinePositions - 1) / kLinePositions;
+}
+
+// 8 threads per line, one 16-byte word each: words 0..3 text, 4..7 ranks. The text is 16-byte
+// aligned and readable to span + 63; the rank array holds 15 words past its last sample.
+__global__ void buildLinesKernel(const uint4* __restrict__ text, const uint32_t* __restrict__ ranks, uint64_t lines, uint4* __restrict__ out) {
+    const uint64_t x = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (x >= lines * 8) {
+        return;
+    }
+    const uint64_t b = x >> 3;
+    const uint32_t t = static_cast<uint32_t>(x & 7);
+    uint4 v;
+    if (t < 4) {
+        // Bytes 32b - 1 + 16t .. 32b + 14 + 16t: the last byte of the aligned chunk before, then 15.
+        const uint64_t chunk = 2 * b + t;  // the aligned chunk at 32b + 16t
+        const uint4 cur = text[chunk];
+        const uint32_t prevLast = chunk > 0 ? text[chunk - 1].w >> 24 : 0;
+        v.x = (cur.x << 8) | prevLast;
+        v.y = (cur.y << 8) | (cur.x >> 24);
+        v.z = (cur.z << 8) | (cur.y >> 24);
+        v.w = (cur.w << 8) | (cur.z >> 24);
+    } else {
+        const uint64_t first = 7 * b + 4 * (t - 4);
+        v.x = ranks[first];
+        v.y = ranks[first + 1];
+        v.z = t < 7 ? ranks[first + 2] : 0;
+        v.w = t < 7 ? ranks[first + 3] : 0;
+    }
+    out[x] = v;
+}
 
 // The sorted lists of one merge level.
 struct ListSet {
@@ -78,21 +139,72 @@ struct MergeRecord<V, false> {
     uint32_t position;
 };
 
-// A record with few, wide gathers: three 16-byte loads hold T[p - 1 .. p + 31] (before = T[p - 1]),
-// and the rank window comes from 16-byte loads at the aligned start of pack(p). Divergent loads cost
-// the texture path one cycle per lane each, so the count of load instructions is what matters.
-// The text buffer and the rank array are 16-byte aligned and padded past their ends.
+// A record with few, wide gathers, in two steps so that a caller can issue the loads early and use
+// them late: fetchRecord issues three 16-byte loads holding T[p - 1 .. p + 31] and kQuads 16-byte
+// loads holding the rank window; decodeRecord shifts them into place. From the lines (cover 32)
+// all of them lie in one 128-byte line; otherwise they come from the text and the rank array
+// (16-byte aligned and padded past their ends). `below` is the number of cover elements below
+// p mod V (read by the caller from whichever copy of the table is closest).
+template <int V>
+struct RawRecord {
+    static constexpr int kQuads = (CoverSet<V>::kSize + 3 + 3) / 4;  // 16-byte loads covering kSize words from any offset
+    uint4 chunk[3];
+    uint4 quad[kQuads];
+};
+
+template <int V>
+__device__ __forceinline__ bool recordFromLines(const MergeCtx& c) {
+    if constexpr (V == kLinePositions) {
+        return c.lines != nullptr;
+    } else {
+        return false;
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void fetchRecord(const MergeCtx& c, uint32_t p, uint32_t below, RawRecord<V>& raw) {
+    const uint4* chunks;
+    const uint4* quads;
+    if (recordFromLines<V>(c)) {
+        const uint4* line = c.lines + (static_cast<uint64_t>(p / kLinePositions) << 3);
+        chunks = line + ((p & (kLinePositions - 1)) >> 4);
+        quads = line + 4 + (below >> 2);
+    } else {
+        const uint32_t first = p > 0 ? p - 1 : 0;
+        chunks = reinterpret_cast<const uint4*>(c.text) + (first >> 4);
+        const uint32_t index = (p / V) * CoverSet<V>::kSize + below;
+        quads = reinterpret_cast<const uint4*>(c.ranks) + (index >> 2);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        raw.chunk[q] = chunks[q];
+    }
+#pragma unroll
+    for (int q = 0; q < RawRecord<V>::kQuads; q++) {
+        raw.quad[q] = quads[q];
+    }
+}
+
 template <int V, class Texts>
-__device__ __forceinline__ void loadRecord(const MergeCtx& c, const Texts& texts, uint32_t p, MergeRecord<V, Texts::kBatch>& r, uint8_t* before = nullptr) {
+__device__ __forceinline__ void decodeRecord(const MergeCtx& c, const Texts& texts, uint32_t p, uint32_t below, const RawRecord<V>& raw, MergeRecord<V, Texts::kBatch>& r, uint8_t* before) {
     static_assert(kRecordWords == 4, "three 16-byte loads cover 32 characters and the one before");
-    const uint32_t first = p > 0 ? p - 1 : 0;
-    const uint32_t base = first & ~15u;
-    const uint4* chunks = reinterpret_cast<const uint4*>(c.text) + (base >> 4);
-    const uint4 q0 = chunks[0], q1 = chunks[1], q2 = chunks[2];
+    constexpr int kSize = CoverSet<V>::kSize;
+    constexpr int kQuads = RawRecord<V>::kQuads;
+    // T[p] sits at byte `offset` of the 48 loaded bytes; the first rank at word `o` of the quads.
+    uint32_t offset, o;
+    if (recordFromLines<V>(c)) {
+        const uint32_t q = p & (kLinePositions - 1);
+        offset = q + 1 - (q & 16);  // 1 .. 16
+        o = below & 3;
+    } else {
+        const uint32_t first = p > 0 ? p - 1 : 0;
+        offset = p - (first & ~15u);  // 0 .. 16
+        o = ((p / V) * kSize + below) & 3;
+    }
+    const uint4 q0 = raw.chunk[0], q1 = raw.chunk[1], q2 = raw.chunk[2];
     uint64_t w[6] = {q0.x | (uint64_t(q0.y) << 32), q0.z | (uint64_t(q0.w) << 32),
                      q1.x | (uint64_t(q1.y) << 32), q1.z | (uint64_t(q1.w) << 32),
                      q2.x | (uint64_t(q2.y) << 32), q2.z | (uint64_t(q2.w) << 32)};
-    const uint32_t offset = p - base;  // 0 .. 16
     if (before) {
         const uint32_t b = offset - 1;  // only meaningful for p > 0
         *before = p > 0 ? static_cast<uint8_t>(w[(b >> 3) & 1] >> ((b & 7) * 8)) : 0;
@@ -112,21 +224,14 @@ __device__ __forceinline__ void loadRecord(const MergeCtx& c, const Texts& texts
     for (int k = 0; k < kRecordWords; k++) {
         r.words[k] = __builtin_bswap64(s ? (v[k] >> s) | (v[k + 1] << (64 - s)) : v[k]);
     }
-
-    constexpr int kSize = CoverSet<V>::kSize;
-    constexpr int kQuads = (kSize + 3 + 3) / 4;  // 16-byte loads covering kSize words from any offset
-    const uint32_t index = packPosition<V>(p);
-    const uint4* quads = reinterpret_cast<const uint4*>(c.ranks) + (index >> 2);
     uint32_t x[4 * kQuads];
 #pragma unroll
     for (int q = 0; q < kQuads; q++) {
-        const uint4 t = quads[q];
-        x[4 * q] = t.x;
-        x[4 * q + 1] = t.y;
-        x[4 * q + 2] = t.z;
-        x[4 * q + 3] = t.w;
+        x[4 * q] = raw.quad[q].x;
+        x[4 * q + 1] = raw.quad[q].y;
+        x[4 * q + 2] = raw.quad[q].z;
+        x[4 * q + 3] = raw.quad[q].w;
     }
-    const uint32_t o = index & 3;
 #pragma unroll
     for (int s2 = 0; s2 < kSize; s2++) {
         r.ranks[s2] = o == 0 ? x[s2] : o == 1 ? x[s2 + 1] : o == 2 ? x[s2 + 2] : x[s2 + 3];
@@ -137,6 +242,14 @@ __device__ __forceinline__ void loadRecord(const MergeCtx& c, const Texts& texts
         const uint32_t remaining = c.segments[segment].end - p;
         r.tag = (segment << 9) | min(remaining, static_cast<uint32_t>(V));
     }
+}
+
+template <int V, class Texts>
+__device__ __forceinline__ void loadRecord(const MergeCtx& c, const Texts& texts, uint32_t p, MergeRecord<V, Texts::kBatch>& r, uint8_t* before = nullptr) {
+    const uint32_t below = deviceCover<V>().below[p & (V - 1)];
+    RawRecord<V> raw;
+    fetchRecord<V>(c, p, below, raw);
+    decodeRecord<V, Texts>(c, texts, p, below, raw, r, before);
 }
 
 template <int V, class Texts>
@@ -443,48 +556,114 @@ inline size_t mergeSharedBytes(uint32_t capacity) {
     return static_cast<size_t>(capacity) * (LdsRecords<V, kBatch>::bytesPerRecord() + 2 * sizeof(uint16_t) + 1) + 16;
 }
 
-// Record x < record y, both in LDS.
-template <int V, class Texts>
-__device__ __forceinline__ bool lessLds(const MergeCtx& c, const TieTables& t, const LdsRecords<V, Texts::kBatch>& R, uint32_t x, uint32_t y) {
-    uint32_t ra, rb;
-    if constexpr (Texts::kBatch) {
-        const uint32_t ta = R.tag[x], tb = R.tag[y];
-        if ((ta >> 9) != (tb >> 9)) {
-            return (ta >> 9) < (tb >> 9);
-        }
-        ra = ta & 0x1FF;
-        rb = tb & 0x1FF;
-    }
-    const uint32_t i = R.position[x], j = R.position[y];
-    if constexpr (!Texts::kBatch) {
-        ra = min(c.n - i, static_cast<uint32_t>(V));
-        rb = min(c.n - j, static_cast<uint32_t>(V));
-    }
-    const uint32_t k = t.tie[(i & (V - 1)) * V + (j & (V - 1))];
-    const uint32_t common = min(k, min(ra, rb));
-    const uint32_t cap = R.capacity;
+// Tiles are latency-bound, so two workgroups per CU matter more than tile size: the default K is
+// the largest whose tiles (at most K*G + C*(G-1) records) leave room for two in the 160 KB of LDS.
+constexpr size_t kTileLdsBudget = 80 * 1024 - 1024;
 
-#pragma unroll
-    for (uint32_t w = 0; w < V / 8; w++) {
-        if (8 * w >= common) {
-            break;
-        }
-        const uint64_t mask = prefixMask(common - 8 * w);
-        const uint64_t a = (w < kRecordWords ? R.words[w * cap + x] : load8(c.text, i + 8 * w)) & mask;
-        const uint64_t b = (w < kRecordWords ? R.words[w * cap + y] : load8(c.text, j + 8 * w)) & mask;
-        if (a != b) {
-            return a < b;
-        }
-    }
-    if (common < k || ra == k || rb == k) {
-        return i > j;  // one is a prefix of the other: the shorter suffix is smaller
-    }
-    return R.ranks[t.slot[(i & (V - 1)) * V + k] * cap + x] < R.ranks[t.slot[(j & (V - 1)) * V + k] * cap + y];
+template <int V, bool kBatch>
+inline uint32_t defaultTileSplitters(uint32_t g) {
+    const size_t perRecord = LdsRecords<V, kBatch>::bytesPerRecord() + 2 * sizeof(uint16_t) + 1;
+    const size_t records = (kTileLdsBudget - 16) / perRecord;
+    const size_t spill = static_cast<size_t>(DifferenceCover<V>::kClasses) * (g - 1);
+    return records > spill + g ? static_cast<uint32_t>((records - spill) / g) : 1;
 }
 
+// Record x < record y, both in LDS. Cover 32 keeps all the characters a comparison can need in
+// the record and finds the tie offset and rank slots by bit arithmetic, so a comparison is at most
+// two dependent LDS round trips: the positions with the first words, then - only if those tie -
+// the other words and the two ranks together.
+template <int V, class Texts>
+__device__ __forceinline__ bool lessLds(const MergeCtx& c, const TieTables& t, const LdsRecords<V, Texts::kBatch>& R, uint32_t x, uint32_t y) {
+    if constexpr (V == 32) {
+        const uint32_t cap = R.capacity;
+        const uint32_t i = R.position[x], j = R.position[y];
+        const uint64_t a0 = R.words[x], b0 = R.words[y];
+        uint32_t ra, rb, sameText = 1, textLess = 0;
+        if constexpr (Texts::kBatch) {
+            const uint32_t ta = R.tag[x], tb = R.tag[y];
+            sameText = (ta >> 9) == (tb >> 9);
+            textLess = (ta >> 9) < (tb >> 9);
+            ra = ta & 0x1FF;
+            rb = tb & 0x1FF;
+        } else {
+            ra = min(c.n - i, static_cast<uint32_t>(V));
+            rb = min(c.n - j, static_cast<uint32_t>(V));
+        }
+        const uint32_t wi = windowOf32(coverMask<V>(), i), wj = windowOf32(coverMask<V>(), j);
+        const uint32_t k = __builtin_ctz(wi & wj);
+        const uint32_t common = min(k, min(ra, rb));
+        const uint64_t m0 = common == 0 ? 0 : prefixMask(common);
+        bool decided = (a0 & m0) != (b0 & m0);
+        bool less = (a0 & m0) < (b0 & m0);
+        if (!decided) {
+            const bool rankTie = common == k && ra != k && rb != k;
+            uint32_t rankA = 0, rankB = 0;
+            if (rankTie) {
+                const uint32_t before = (1u << k) - 1;
+                rankA = R.ranks[__builtin_popcount(wi & before) * cap + x];
+                rankB = R.ranks[__builtin_popcount(wj & before) * cap + y];
+            }
+            if (common > 8) {
+#pragma unroll
+                for (int w = 1; w < kRecordWords; w++) {
+                    const uint32_t rest = common > 8u * w ? common - 8u * w : 0u;
+                    const uint64_t mask = rest == 0 ? 0 : prefixMask(rest);
+                    const uint64_t av = R.words[w * cap + x] & mask, bv = R.words[w * cap + y] & mask;
+                    less = decided ? less : av < bv;
+                    decided = decided || av != bv;
+                }
+            }
+            if (!decided) {
+                less = rankTie ? rankA < rankB : i > j;  // i > j: one is a prefix of the other, the shorter is smaller
+            }
+        }
+        return sameText ? less : textLess;
+    } else {
+        uint32_t ra, rb;
+        if constexpr (Texts::kBatch) {
+            const uint32_t ta = R.tag[x], tb = R.tag[y];
+            if ((ta >> 9) != (tb >> 9)) {
+                return (ta >> 9) < (tb >> 9);
+            }
+            ra = ta & 0x1FF;
+            rb = tb & 0x1FF;
+        }
+        const uint32_t i = R.position[x], j = R.position[y];
+        if constexpr (!Texts::kBatch) {
+            ra = min(c.n - i, static_cast<uint32_t>(V));
+            rb = min(c.n - j, static_cast<uint32_t>(V));
+        }
+        const uint32_t k = t.tie[(i & (V - 1)) * V + (j & (V - 1))];
+        const uint32_t common = min(k, min(ra, rb));
+        const uint32_t cap = R.capacity;
+
+#pragma unroll
+        for (uint32_t w = 0; w < V / 8; w++) {
+            if (8 * w >= common) {
+                break;
+            }
+            const uint64_t mask = prefixMask(common - 8 * w);
+            const uint64_t a = (w < kRecordWords ? R.words[w * cap + x] : load8(c.text, i + 8 * w)) & mask;
+            const uint64_t b = (w < kRecordWords ? R.words[w * cap + y] : load8(c.text, j + 8 * w)) & mask;
+            if (a != b) {
+                return a < b;
+            }
+        }
+        if (common < k || ra == k || rb == k) {
+            return i > j;  // one is a prefix of the other: the shorter suffix is smaller
+        }
+        return R.ranks[t.slot[(i & (V - 1)) * V + k] * cap + x] < R.ranks[t.slot[(j & (V - 1)) * V + k] * cap + y];
+    }
+}
+
+// Outputs [o, o1) of one round of pairwise merges: run 2p and run 2p + 1 of src (bounds[0..runs])
+// merge into dst at the same place. Each output range starts with a merge-path search for its
+// first output, then merges sequentially with the two head indices in registers and the indices
+// after them loaded together with each comparison. src holds `capacity` entries.
 template <int V, class Texts>
 __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const TieTables& t, const LdsRecords<V, Texts::kBatch>& recs, const uint16_t* src, uint16_t* dst, const uint32_t* bounds, uint32_t runs, uint32_t o, uint32_t o1) {
     const uint32_t pairs = (runs + 1) / 2;
+    const uint32_t last = recs.capacity - 1;
     uint32_t j = 0;
     while (o < o1) {
         while (j + 1 < pairs && bounds[2 * (j + 1)] <= o) {
@@ -495,6 +674,8 @@ __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const TieTables& t
         const uint32_t bEnd = bounds[min(2 * j + 2, runs)];
         const uint32_t lengthA = aEnd - aStart, lengthB = bEnd - aEnd;
         const uint32_t d = o - aStart;
+        // The number of A elements among the first d outputs: the first m in [lo, hi] for which
+        // A[m] < B[d - 1 - m] is false (hi when there is none).
         uint32_t lo = d > lengthB ? d - lengthB : 0;
         uint32_t hi = min(d, lengthA);
         while (lo < hi) {
@@ -510,20 +691,105 @@ __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const TieTables& t
         if (stop <= o) {
             return;  // inconsistent bounds; the output check reports it
         }
+        uint16_t headA = src[min(aStart + ia, last)], headB = src[min(aEnd + ib, last)];
         while (o < stop) {
+            const uint16_t nextA = src[min(aStart + ia + 1, last)];
+            const uint16_t nextB = src[min(aEnd + ib + 1, last)];
             bool takeA;
             if (ib >= lengthB) {
                 takeA = true;
             } else if (ia >= lengthA) {
                 takeA = false;
             } else {
-                takeA = lessLds<V, Texts>(c, t, recs, src[aStart + ia], src[aEnd + ib]);
+                takeA = lessLds<V, Texts>(c, t, recs, headA, headB);
             }
-            dst[o++] = takeA ? src[aStart + ia++] : src[aEnd + ib++];
+            dst[o++] = takeA ? headA : headB;
+            if (takeA) {
+                ia++;
+                headA = nextA;
+            } else {
+                ib++;
+                headB = nextB;
+            }
         }
     }
 }
 
+// The comparison tables of a tile merge for covers other than 32 (cover 32 computes them).
+template <int V>
+__device__ __forceinline__ TieTables setupTables() {
+    const CoverTables<V>& cv = deviceCover<V>();
+    return TieTables{&cv.tie[0][0], &cv.slot[0][0]};
+}
+
+// Merges the runs of indices in src (run starts bounds[0..runs), bounds[runs] = size) in LDS by
+// rounds of pairwise merges; returns the buffer holding the merged order. afterRound() runs after
+// every round, between barriers, on every thread.
+template <int V, class Texts, int kThreads, class AfterRound>
+__device__ __forceinline__ uint16_t* mergeRuns(const MergeCtx& c, const TieTables& tables, const LdsRecords<V, Texts::kBatch>& recs, uint16_t* src, uint16_t* dst, uint32_t* bounds, uint32_t runs, uint32_t size, AfterRound afterRound) {
+    const uint32_t chunk = max(4u, (size + kThreads - 1) / kThreads);
+    while (runs > 1) {
+        for (uint64_t o = static_cast<uint64_t>(threadIdx.x) * chunk; o < size; o += static_cast<uint64_t>(kThreads) * chunk) {
+            mergeChunk<V, Texts>(c, tables, recs, src, dst, bounds, runs, static_cast<uint32_t>(o), static_cast<uint32_t>(min(o + chunk, static_cast<uint64_t>(size))));
+        }
+        __syncthreads();
+        const uint32_t pairs = (runs + 1) / 2;
+        if (threadIdx.x == 0) {
+            for (uint32_t j = 0; j < pairs; j++) {
+                bounds[j] = bounds[2 * j];
+            }
+            bounds[pairs] = size;
+        }
+        __syncthreads();
+        uint16_t* t = src;
+        src = dst;
+        dst = t;
+        runs = pairs;
+        afterRound();
+    }
+    return src;
+}
+
+// Writes a merged tile: order[r] is the record of output row outBase + r.
+template <int V, class Texts, int kFlags, int kThreads>
+__device__ __forceinline__ void writeTile(const MergeCtx& c, const ListSet& lists, const LdsRecords<V, Texts::kBatch>& recs, const uint16_t* order, const uint8_t* before, const uint32_t* segStart, const uint32_t* rowStart, uint32_t outBase, uint32_t size, const TileOut& out, uint32_t* errors) {
+    const int lc = lists.count;
+    for (uint32_t r = threadIdx.x; r < size; r += kThreads) {
+        const uint32_t e = order[r];
+        const uint32_t p = recs.position[e];
+        const uint32_t row = outBase + r;
+        uint32_t start = 0, rowBase = 0, segment = 0;
+        if constexpr (Texts::kBatch) {
+            segment = recs.tag[e] >> 9;
+            start = c.segments[segment].start;
+            rowBase = c.segments[segment].rowBase;
+        }
+        if constexpr ((kFlags & kOutSA) != 0) {
+            out.sa[row] = p - start;
+        }
+        if constexpr ((kFlags & kOutIndex) != 0) {
+            int l = 0;
+            while (l + 1 < lc && e >= segStart[l + 1]) {
+                l++;
+            }
+            out.index[row] = lists.indexBase[l] + rowStart[l] + (e - segStart[l]);
+        }
+        if constexpr ((kFlags & kOutBWT) != 0) {
+            if (p != start) {
+                const uint32_t local = row - rowBase;
+                const uint32_t first = out.firstSuffixRow[segment];
+                const uint32_t target = rowBase + (local < first ? local + 1 : local);
+                if (target < out.bwtLength) {
+                    out.bwt[target] = before[e];
+                } else {
+                    atomicOr(errors, 4u);
+                }
+            }
+        }
+    }
+}
+
+// One tile per workgroup: load its records into LDS, merge, write.
 template <int V, class Texts, int kFlags>
 __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Texts texts, ListSet lists, const uint32_t* __restrict__ rows, uint32_t capacity, TileOut out, uint32_t* __restrict__ errors) {
     using Record = MergeRecord<V, Texts::kBatch>;
@@ -537,28 +803,12 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
     __shared__ uint32_t segStart[kMaxLists + 1];
     __shared__ uint32_t rowStart[kMaxLists];
     __shared__ uint32_t outBase;
-    constexpr bool kTablesInLds = V <= 32;
-    __shared__ uint8_t ldsTie[kTablesInLds ? V * V : 1];
-    __shared__ uint8_t ldsSlot[kTablesInLds ? V * V : 1];
+    __shared__ bool valid;
 
     const int lc = lists.count;
     const uint64_t tile = blockIdx.x;
+    const TieTables tables = setupTables<V>();
 
-    TieTables tables;
-    if constexpr (kTablesInLds) {
-        const CoverTables<V>& cv = deviceCover<V>();
-        for (uint32_t x = threadIdx.x; x < V * V; x += kMergeThreads) {
-            ldsTie[x] = (&cv.tie[0][0])[x];
-            ldsSlot[x] = (&cv.slot[0][0])[x];
-        }
-        tables.tie = ldsTie;
-        tables.slot = ldsSlot;
-    } else {
-        tables.tie = &deviceCover<V>().tie[0][0];
-        tables.slot = &deviceCover<V>().slot[0][0];
-    }
-
-    __shared__ bool valid;
     if (threadIdx.x == 0) {
         valid = tileLayout(lists, rows, tile, segStart, rowStart, outBase) && segStart[lc] <= capacity;
         for (int l = 0; l <= lc; l++) {
@@ -616,63 +866,8 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
     }
     __syncthreads();
 
-    uint16_t* src = indexA;
-    uint16_t* dst = indexB;
-    uint32_t runs = static_cast<uint32_t>(lc);
-    const uint32_t chunk = max(4u, (size + kMergeThreads - 1) / kMergeThreads);
-
-    while (runs > 1) {
-        for (uint64_t o = static_cast<uint64_t>(threadIdx.x) * chunk; o < size; o += static_cast<uint64_t>(kMergeThreads) * chunk) {
-            mergeChunk<V, Texts>(c, tables, recs, src, dst, bounds, runs, static_cast<uint32_t>(o), static_cast<uint32_t>(min(o + chunk, static_cast<uint64_t>(size))));
-        }
-        __syncthreads();
-        const uint32_t pairs = (runs + 1) / 2;
-        if (threadIdx.x == 0) {
-            for (uint32_t j = 0; j < pairs; j++) {
-                bounds[j] = bounds[2 * j];
-            }
-            bounds[pairs] = size;
-        }
-        __syncthreads();
-        uint16_t* t = src;
-        src = dst;
-        dst = t;
-        runs = pairs;
-    }
-
-    for (uint32_t r = threadIdx.x; r < size; r += kMergeThreads) {
-        const uint32_t e = src[r];
-        const uint32_t p = recs.position[e];
-        const uint32_t row = outBase + r;
-        uint32_t start = 0, rowBase = 0, segment = 0;
-        if constexpr (Texts::kBatch) {
-            segment = recs.tag[e] >> 9;
-            start = c.segments[segment].start;
-            rowBase = c.segments[segment].rowBase;
-        }
-        if constexpr ((kFlags & kOutSA) != 0) {
-            out.sa[row] = p - start;
-        }
-        if constexpr ((kFlags & kOutIndex) != 0) {
-            int l = 0;
-            while (l + 1 < lc && e >= segStart[l + 1]) {
-                l++;
-            }
-            out.index[row] = lists.indexBase[l] + rowStart[l] + (e - segStart[l]);
-        }
-        if constexpr ((kFlags & kOutBWT) != 0) {
-            if (p != start) {
-                const uint32_t local = row - rowBase;
-                const uint32_t first = out.firstSuffixRow[segment];
-                const uint32_t target = rowBase + (local < first ? local + 1 : local);
-                if (target < out.bwtLength) {
-                    out.bwt[target] = before[e];
-                } else {
-                    atomicOr(errors, 4u);
-                }
-            }
-        }
-    }
+    const uint16_t* order = mergeRuns<V, Texts, kMergeThreads>(c, tables, recs, indexA, indexB, bounds, static_cast<uint32_t>(lc), size, [] {});
+    writeTile<V, Texts, kFlags, kMergeThreads>(c, lists, recs, order, before, segStart, rowStart, outBase, size, out, errors);
 }
 
 // Per text: the row of its first suffix (the number of its suffixes smaller than it), and the

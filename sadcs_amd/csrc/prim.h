@@ -358,211 +358,8 @@ __global__ __launch_bounds__(kSortThreads) void radixDownsweepKernel(Source sour
     }
 }
 
-// ==== one pass per digit: decoupled look-back ====================================================
-//
-// The digit counts of every pass come from one read of the keys (or from whoever produced them).
-// Each pass is then one kernel: a workgroup takes the next tile number from a counter (so every
-// earlier tile already runs on the device), ranks its keys as above, publishes its per-digit
-// counts, and adds up the counts of earlier tiles by walking back until it meets a tile that
-// published its inclusive prefix. A status word is (epoch, flag, count) in 64 bits: entries of
-// another pass or stale memory never match the epoch. The walk is bounded: past kLookbackSpins
-// polls of one entry it raises an error flag and stops waiting, so a fault of this logic shows
-// as a failed construction, never as a hang.
-
-constexpr uint32_t kLookbackSpins = 1u << 20;
-constexpr uint64_t kStatusAggregate = 1;
-constexpr uint64_t kStatusInclusive = 2;
-constexpr int kMaxDigitPasses = 8;
-
-__device__ __forceinline__ uint64_t packStatus(uint32_t epoch, uint64_t flag, uint32_t count) {
-    return (uint64_t(epoch) << 34) | (flag << 32) | count;
-}
-
-// Digit counts of every pass [pass][digit] over the kept items of the source.
-template <class K, class Source>
-__global__ __launch_bounds__(256) void radixCountAllKernel(Source source, uint32_t count, int beginBit, int endBit, uint32_t* __restrict__ counts) {
-    __shared__ uint32_t local[kMaxDigitPasses][kRadixBins];
-    const int passes = (endBit - beginBit + kRadixBits - 1) / kRadixBits;
-    for (uint32_t x = threadIdx.x; x < kMaxDigitPasses * kRadixBins; x += blockDim.x) {
-        (&local[0][0])[x] = 0;
-    }
-    __syncthreads();
-    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x) {
-        K key = 0;
-        uint32_t value = 0;
-        if (source(static_cast<uint32_t>(i), key, value)) {
-            for (int p = 0; p < passes; p++) {
-                const int shift = beginBit + p * kRadixBits;
-                const int width = min(kRadixBits, endBit - shift);
-                atomicAdd(&local[p][static_cast<uint32_t>(key >> shift) & ((1u << width) - 1)], 1u);
-            }
-        }
-    }
-    __syncthreads();
-    for (uint32_t x = threadIdx.x; x < passes * kRadixBins; x += blockDim.x) {
-        const uint32_t c = (&local[0][0])[x];
-        if (c) {
-            atomicAdd(&counts[x], c);
-        }
-    }
-}
-
-// counts[pass][digit] -> exclusive starts per pass, in place; status entries and tile counters cleared.
-__global__ __launch_bounds__(kRadixBins) void radixStartsKernel(uint32_t* __restrict__ counts, int passes, uint64_t* __restrict__ status, uint64_t statusWords, uint32_t* __restrict__ tileCounters) {
-    __shared__ uint32_t waveSums[kRadixBins / kWave];
-    const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    for (int p = 0; p < passes; p++) {
-        const uint32_t c = counts[p * kRadixBins + threadIdx.x];
-        const uint32_t inclusive = waveInclusive<OpSum>(c);
-        if (lane == kWave - 1) {
-            waveSums[wave] = inclusive;
-        }
-        __syncthreads();
-        uint32_t prefix = 0;
-        for (uint32_t w = 0; w < wave; w++) {
-            prefix += waveSums[w];
-        }
-        counts[p * kRadixBins + threadIdx.x] = prefix + inclusive - c;
-        __syncthreads();
-    }
-    if (threadIdx.x < kMaxDigitPasses) {
-        tileCounters[threadIdx.x] = 0;
-    }
-    (void)status;
-    (void)statusWords;
-}
-
-__global__ void clearStatusKernel(uint64_t* __restrict__ status, uint64_t words) {
-    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < words; i += uint64_t(gridDim.x) * blockDim.x) {
-        status[i] = 0;
-    }
-}
-
-template <class K, class Source>
-__global__ __launch_bounds__(kSortThreads) void radixOnesweepKernel(Source source, uint32_t count, int shift, uint32_t mask, const uint32_t* __restrict__ starts, uint64_t* __restrict__ status, uint32_t* __restrict__ tileCounter, uint32_t epoch, K* __restrict__ keysOut, uint32_t* __restrict__ valuesOut, uint32_t limit, uint32_t* __restrict__ errors) {
-    __shared__ uint32_t waveCounts[kSortWaves][kRadixBins];
-    __shared__ uint32_t digitStart[kRadixBins];
-    __shared__ uint32_t globalStart[kRadixBins];
-    __shared__ uint32_t waveSums[kRadixBins / kWave];
-    __shared__ K sortedKeys[kSortTile];
-    __shared__ uint32_t sortedValues[kSortTile];
-    __shared__ uint32_t tileShared;
-
-    const uint32_t wave = threadIdx.x / kWave;
-    const uint32_t lane = threadIdx.x % kWave;
-    if (threadIdx.x == 0) {
-        tileShared = atomicAdd(tileCounter, 1u);
-    }
-    for (uint32_t x = threadIdx.x; x < kSortWaves * kRadixBins; x += kSortThreads) {
-        (&waveCounts[0][0])[x] = 0;
-    }
-    __syncthreads();
-    const uint32_t tile = tileShared;
-
-    const uint64_t base = static_cast<uint64_t>(tile) * kSortTile + wave * kSortItems * kWave + lane;
-    K keys[kSortItems];
-    uint32_t values[kSortItems];
-    uint32_t ranks[kSortItems];
-    bool valids[kSortItems];
-
-#pragma unroll
-    for (int j = 0; j < kSortItems; j++) {
-        const uint64_t i = base + static_cast<uint64_t>(j) * kWave;
-        keys[j] = 0;
-        values[j] = 0;
-        valids[j] = i < count && source(static_cast<uint32_t>(i), keys[j], values[j]);
-        const uint32_t digit = digitOf(keys[j], shift, mask);
-        const uint64_t peers = matchDigit<kRadixBits>(digit, valids[j]);
-        const uint32_t before = valids[j] ? waveCounts[wave][digit] : 0;
-        ranks[j] = before + countBelow(peers);
-        if (valids[j] && lane == static_cast<uint32_t>(__builtin_ctzll(peers))) {
-            waveCounts[wave][digit] = before + static_cast<uint32_t>(__builtin_popcountll(peers));
-        }
-    }
-    __syncthreads();
-
-    uint32_t total = 0;
-    if (threadIdx.x < kRadixBins) {
-        const uint32_t d = threadIdx.x;
-        for (int w = 0; w < kSortWaves; w++) {
-            const uint32_t c = waveCounts[w][d];
-            waveCounts[w][d] = total;
-            total += c;
-        }
-        const uint32_t inclusive = waveInclusive<OpSum>(total);
-        if (lane == kWave - 1) {
-            waveSums[wave] = inclusive;
-        }
-        digitStart[d] = inclusive - total;
-
-        // Publish, then walk back over the earlier tiles for this digit.
-        uint64_t* mine = status + static_cast<uint64_t>(tile) * kRadixBins + d;
-        uint32_t prefix = 0;
-        if (tile == 0) {
-            __hip_atomic_store(mine, packStatus(epoch, kStatusInclusive, total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(mine, packStatus(epoch, kStatusAggregate, total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t t = static_cast<int64_t>(tile) - 1;
-            uint32_t spins = 0;
-            while (t >= 0) {
-                const uint64_t v = __hip_atomic_load(status + static_cast<uint64_t>(t) * kRadixBins + d, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                const uint64_t flag = (v >> 32) & 3;
-                if ((v >> 34) == epoch && flag != 0) {
-                    prefix += static_cast<uint32_t>(v);
-                    if (flag == kStatusInclusive) {
-                        break;
-                    }
-                    t--;
-                    spins = 0;
-                } else if (++spins > kLookbackSpins) {
-                    atomicOr(errors, 8u);
-                    break;
-                } else {
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
-            __hip_atomic_store(mine, packStatus(epoch, kStatusInclusive, prefix + total), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        globalStart[d] = starts[d] + prefix;
-    }
-    __syncthreads();
-    if (threadIdx.x < kRadixBins) {
-        uint32_t prefix = 0;
-        for (uint32_t w = 0; w < wave; w++) {
-            prefix += waveSums[w];
-        }
-        digitStart[threadIdx.x] += prefix;
-    }
-    __syncthreads();
-
-#pragma unroll
-    for (int j = 0; j < kSortItems; j++) {
-        if (valids[j]) {
-            const uint32_t digit = digitOf(keys[j], shift, mask);
-            const uint32_t local = digitStart[digit] + waveCounts[wave][digit] + ranks[j];
-            sortedKeys[local] = keys[j];
-            sortedValues[local] = values[j];
-        }
-    }
-    __syncthreads();
-
-    uint32_t kept = 0;
-    for (int w = 0; w < kRadixBins / kWave; w++) {
-        kept += waveSums[w];
-    }
-    for (uint32_t x = threadIdx.x; x < kept; x += kSortThreads) {
-        const K key = sortedKeys[x];
-        const uint32_t digit = digitOf(key, shift, mask);
-        const uint32_t target = globalStart[digit] + (x - digitStart[digit]);
-        if (target < limit) {
-            keysOut[target] = key;
-            valuesOut[target] = sortedValues[x];
-        }
-    }
-}
-
-// Buffers and scratch of a sort over up to `capacity` items. With `status` set, sorts run one
-// kernel per digit pass (look-back); otherwise reduce-then-scan (histogram, scan, scatter).
+// Buffers and scratch of a sort over up to `capacity` items (reduce-then-scan per digit pass:
+// tile histograms, one scan, scatter).
 template <class K>
 struct SortBuffers {
     K* keys[2] = {};
@@ -570,15 +367,7 @@ struct SortBuffers {
     int current = 0;        // which buffer holds the data
     uint32_t* hist = nullptr;
     Sum1* aggregates = nullptr;
-    uint64_t* status = nullptr;       // tiles * 256 status words
-    uint32_t* digitCounts = nullptr;  // kMaxDigitPasses * 256
-    uint32_t* tileCounters = nullptr; // kMaxDigitPasses
-    uint32_t* errors = nullptr;
 };
-
-inline size_t radixStatusWords(uint64_t capacity) {
-    return static_cast<size_t>(std::max<uint32_t>(1, ceilDiv(capacity, kSortTile))) * kRadixBins;
-}
 
 inline size_t radixHistWords(uint64_t capacity) {
     return static_cast<size_t>(std::max<uint32_t>(1, ceilDiv(capacity, kSortTile))) * kRadixBins;
@@ -601,43 +390,10 @@ inline void radixPass(hipStream_t stream, SortBuffers<K>& b, Source source, uint
     b.current = 1 - b.current;
 }
 
-// Look-back sort: digit counts of all passes from the first source, then one kernel per pass.
-// The first pass reads `first`, later passes the buffers; the result lands in b.keys[b.current].
-template <class K, class Source>
-inline void radixSortLookback(hipStream_t stream, SortBuffers<K>& b, Source first, uint32_t count, int beginBit, int endBit, uint32_t limit = 0xFFFFFFFFu) {
-    const int passes = (endBit - beginBit + kRadixBits - 1) / kRadixBits;
-    const uint32_t tiles = std::max<uint32_t>(1, ceilDiv(count, kSortTile));
-    fillKernelWords(stream, b.digitCounts, kMaxDigitPasses * kRadixBins);
-    clearStatusKernel<<<std::min<uint32_t>(tiles, 2048), 256, 0, stream>>>(b.status, static_cast<uint64_t>(tiles) * kRadixBins);
-    radixCountAllKernel<K, Source><<<std::min<uint32_t>(std::max<uint32_t>(1, ceilDiv(count, 4096)), 1024), 256, 0, stream>>>(first, count, beginBit, endBit, b.digitCounts);
-    radixStartsKernel<<<1, kRadixBins, 0, stream>>>(b.digitCounts, passes, b.status, 0, b.tileCounters);
-    for (int p = 0; p < passes; p++) {
-        const int shift = beginBit + p * kRadixBits;
-        const int width = std::min(kRadixBits, endBit - shift);
-        const uint32_t mask = (1u << width) - 1;
-        const uint32_t out = limit == 0xFFFFFFFFu ? count : limit;
-        // Epochs differ per pass, so the status words need clearing only once per sort.
-        if (p == 0) {
-            radixOnesweepKernel<K, Source><<<tiles, kSortThreads, 0, stream>>>(first, count, shift, mask, b.digitCounts, b.status,
-                b.tileCounters + p, static_cast<uint32_t>(p + 1), b.keys[1 - b.current], b.values[1 - b.current], out, b.errors);
-        } else {
-            const uint32_t kept = std::min(count, out);
-            radixOnesweepKernel<K, ArraySource<K>><<<std::max<uint32_t>(1, ceilDiv(kept, kSortTile)), kSortThreads, 0, stream>>>(
-                ArraySource<K>{b.keys[b.current], b.values[b.current]}, kept, shift, mask, b.digitCounts + p * kRadixBins, b.status, b.tileCounters + p,
-                static_cast<uint32_t>(p + 1), b.keys[1 - b.current], b.values[1 - b.current], out, b.errors);
-        }
-        b.current = 1 - b.current;
-    }
-}
-
 // Stable sort of b.keys[current], b.values[current] (count items) by bits [beginBit, endBit).
 template <class K>
 inline void radixSort(hipStream_t stream, SortBuffers<K>& b, uint32_t count, int beginBit, int endBit) {
     if (count <= 1 || endBit <= beginBit) {
-        return;
-    }
-    if (b.status) {
-        radixSortLookback<K>(stream, b, ArraySource<K>{b.keys[b.current], b.values[b.current]}, count, beginBit, endBit);
         return;
     }
     for (int shift = beginBit; shift < endBit; shift += kRadixBits) {
