@@ -14,7 +14,14 @@ Printed as one JSON line, with:
     SA + BWT) over the step time, against 8 TB/s;
   cpu_baseline: the reference's CPU construction (oracle/_ref, OpenMP on all host cores) timed
     on a 256 MiB prefix of the same generator's text, SA and BWT built separately as the reference
-    does, MB/s = prefix bytes / (SA seconds + BWT seconds).
+    does, MB/s = prefix bytes / (SA seconds + BWT seconds);
+  batched: 4096 texts of log-uniform length 1 KiB - 4 MiB (even: DNA-like, odd: Markov), all
+    resident in HBM, built in one batched construction, against the same texts built one at a
+    time by the single-text device path; per-text outputs of both compared on a sample;
+  served: requests (lognormal lengths, median 256 KiB, sigma 1.5, capped at 64 MiB; 45% ACGT,
+    45% protein-like, 10% periodic) arriving as a Poisson stream at 0.5, 0.8 and 1.1 times the
+    measured one-at-a-time capacity, through the continuous-batching server (host memory in and
+    out); p50/p99 latency from submission to result and MB/s over the run.
 """
 import argparse
 import json
@@ -53,6 +60,137 @@ def cpu_baseline(text_prefix):
             "sample": f"first {n >> 20} MiB of the same text; reference CPU build (cover 128), SA {t1 - t0:.2f}s + BWT {t2 - t1:.2f}s"}
 
 
+def batched_measurement(count, steps):
+    """Batched construction of `count` mixed texts against one-at-a-time device builds."""
+    import numpy as np
+
+    import sadcs_amd
+    from sadcs_amd import workloads
+
+    t0 = time.perf_counter()
+    texts = workloads.batch_texts(count)
+    lengths = np.array([t.size for t in texts], dtype=np.int64)
+    total = int(lengths.sum())
+    offsets, span = sadcs_amd.gpu_batch_layout(lengths)
+    host = np.zeros(span + 64, dtype=np.uint8)
+    for t, o in zip(texts, offsets):
+        host[o:o + t.size] = t
+    # The one-at-a-time path needs 64 zero bytes after each text.
+    seqOffsets = np.zeros(count, dtype=np.int64)
+    position = 0
+    for i, n in enumerate(lengths):
+        seqOffsets[i] = position
+        position += (int(n) + 64 + 255) // 256 * 256
+    seqHost = np.zeros(position + 64, dtype=np.uint8)
+    for t, o in zip(texts, seqOffsets):
+        seqHost[o:o + t.size] = t
+    rows = np.concatenate([[0], np.cumsum(lengths)[:-1]])
+    print(f"# batched: {count} texts, {total} bytes, generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    d_texts = sadcs_amd.DeviceBuffer.from_host(host)
+    d_seq = sadcs_amd.DeviceBuffer.from_host(seqHost)
+    del host, seqHost
+    d_sa = sadcs_amd.DeviceBuffer(4 * total)
+    d_bwt = sadcs_amd.DeviceBuffer(total)
+    d_sa2 = sadcs_amd.DeviceBuffer(4 * total)
+    d_bwt2 = sadcs_amd.DeviceBuffer(total)
+    ctx = sadcs_amd.GpuContext(int(lengths.max()))
+    batch_ctx = sadcs_amd.GpuContext(0, batch_lengths=lengths)
+    stats = sadcs_amd.GpuStats()
+
+    batch_primaries = sadcs_amd.gpu_batch_build_device(batch_ctx, d_texts.ptr, lengths, d_sa.ptr, d_bwt.ptr)  # warmup
+    sadcs_amd.gpu_synchronize()
+    start = time.perf_counter()
+    for _ in range(steps):
+        batch_primaries = sadcs_amd.gpu_batch_build_device(batch_ctx, d_texts.ptr, lengths, d_sa.ptr, d_bwt.ptr)
+    sadcs_amd.gpu_synchronize()
+    batch_seconds = (time.perf_counter() - start) / steps
+
+    seq_primaries = np.empty(count, dtype=np.int64)
+    sadcs_amd.gpu_synchronize()
+    start = time.perf_counter()
+    for i in range(count):
+        seq_primaries[i] = sadcs_amd.gpu_build_device(ctx, d_seq.ptr + int(seqOffsets[i]), int(lengths[i]),
+                                                      d_sa2.ptr + 4 * int(rows[i]), d_bwt2.ptr + int(rows[i]), stats)
+    sadcs_amd.gpu_synchronize()
+    seq_seconds = time.perf_counter() - start
+    ctx.close()
+    batch_ctx.close()
+
+    # Every text: the batched outputs equal the one-at-a-time outputs.
+    same = bool(np.array_equal(np.asarray(batch_primaries), seq_primaries))
+    for a, b in ((d_sa, d_sa2), (d_bwt, d_bwt2)):
+        same = same and np.array_equal(a.to_host(np.uint8), b.to_host(np.uint8))
+    for buf in (d_texts, d_seq, d_sa, d_bwt, d_sa2, d_bwt2):
+        buf.free()
+    return {"texts": count, "bytes": total, "mb_per_s": round(total / batch_seconds / 1e6, 1),
+            "ms": round(batch_seconds * 1e3, 2), "sequential_mb_per_s": round(total / seq_seconds / 1e6, 1),
+            "sequential_ms": round(seq_seconds * 1e3, 1), "speedup_vs_sequential": round(seq_seconds / batch_seconds, 2),
+            "outputs_equal_sequential": same, "resident": "texts in HBM, outputs in HBM"}
+
+
+def served_measurement(requests, factors=(0.5, 0.8, 1.1)):
+    """Poisson request streams through the serving loop, against one-at-a-time host builds."""
+    import threading
+
+    import numpy as np
+
+    import sadcs_amd
+    from sadcs_amd import workloads
+
+    lengths, kinds = workloads.serving_requests(requests, seed=5)
+    texts = [workloads.serving_text(int(n), int(k), 1000 + i) for i, (n, k) in enumerate(zip(lengths, kinds))]
+    total = int(lengths.sum())
+
+    # One at a time through the host API (the sequential capacity the rates are set against).
+    with sadcs_amd.GpuContext(int(lengths.max())) as ctx:
+        sadcs_amd.gpu_build(texts[0], True, True, context=ctx)
+        start = time.perf_counter()
+        for t in texts:
+            sadcs_amd.gpu_build(t, True, True, context=ctx)
+        seq_seconds = time.perf_counter() - start
+    capacity = len(texts) / seq_seconds
+    print(f"# served: {requests} requests, {total} bytes; one at a time {seq_seconds:.2f}s ({capacity:.0f} requests/s)",
+          file=sys.stderr, flush=True)
+
+    rates = []
+    rng = np.random.default_rng(7)
+    for factor in factors:
+        rate = factor * capacity
+        gaps = rng.exponential(1.0 / rate, len(texts))
+        results = [None] * len(texts)
+        with sadcs_amd.Server(workers=2, max_batch_bytes=64 << 20) as server:
+            ids = [None] * len(texts)
+
+            def waiter():
+                for i in range(len(texts)):
+                    while ids[i] is None:
+                        time.sleep(0.0002)
+                    results[i] = server.wait(ids[i])
+
+            thread = threading.Thread(target=waiter)
+            thread.start()
+            start = time.perf_counter()
+            due = start
+            for i, t in enumerate(texts):
+                due += gaps[i]
+                pause = due - time.perf_counter()
+                if pause > 0:
+                    time.sleep(pause)
+                ids[i] = server.submit(t)
+            thread.join()
+            elapsed = time.perf_counter() - start
+            counters = server.counters()
+        latency = np.array([r[3] for r in results]) * 1e3
+        rates.append({"offered_x_sequential": factor, "offered_requests_per_s": round(rate, 1),
+                      "mb_per_s": round(total / elapsed / 1e6, 1), "p50_ms": round(float(np.percentile(latency, 50)), 2),
+                      "p99_ms": round(float(np.percentile(latency, 99)), 2),
+                      "mean_batch": round(float(np.mean([r[5] for r in results])), 2), "batches": counters["batches"]})
+        print(f"# served at {factor}x: {rates[-1]}", file=sys.stderr, flush=True)
+    return {"requests": requests, "bytes": total, "sequential_mb_per_s": round(total / seq_seconds / 1e6, 1),
+            "sequential_requests_per_s": round(capacity, 1), "rates": rates, "io": "host memory in and out (PCIe)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +200,8 @@ def main():
     ap.add_argument("--cover", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-mib", type=int, default=256)
+    ap.add_argument("--batch-texts", type=int, default=4096, help="texts in the batched measurement (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=300, help="requests per offered rate (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device to device three times (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -145,6 +285,10 @@ def main():
         baseline = cpu_baseline(text[:sample])
 
     ctx.close()
+    for buf in (d_text, d_sa, d_bwt):
+        buf.free()
+    batched = batched_measurement(args.batch_texts, 3) if args.batch_texts else None
+    served = served_measurement(args.serve_requests) if args.serve_requests else None
     ranking, classes, merge = (sum(p[i] for p in phases) / len(phases) for i in range(3))
     line = {
         "metric": METRIC,
@@ -169,6 +313,8 @@ def main():
         "phases_ms": {"ranking": round(ranking * 1e3, 2), "classes": round(classes * 1e3, 2), "merge": round(merge * 1e3, 2)},
         "peak_device_bytes_per_input_byte": round(peak_bytes / n, 3),
         "cpu_baseline": baseline,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -123,6 +123,11 @@ int sadcs_hip_batch_build(sadcs_ctx* context, const uint8_t* texts, const int64_
 /* Device bytes a batch of these lengths needs. */
 int64_t sadcs_hip_batch_required_bytes(const int64_t* lengths, int32_t count);
 
+/* A context whose device memory is planned for a batch of these lengths, so that repeated batch
+ * builds (of this layout or any whose plan fits) allocate nothing. Same refusal rule as
+ * sadcs_hip_ctx_create; single-text builds up to the longest length also fit. */
+int sadcs_hip_ctx_create_batch(const int64_t* lengths, int32_t count, const sadcs_params* params, sadcs_ctx** context);
+
 /* The device layout of a batch: text i starts at offsets[i] (a multiple of 256). Returns the span;
  * a device buffer for the batch holds span bytes plus 64 zero bytes. */
 int64_t sadcs_hip_batch_layout(const int64_t* lengths, int32_t count, int64_t* offsets);

@@ -104,6 +104,8 @@ def _load():
     lib.sadcs_hip_server_counters.restype = None
     lib.sadcs_hip_server_destroy.argtypes = [P]
     lib.sadcs_hip_server_destroy.restype = None
+    lib.sadcs_hip_ctx_create_batch.argtypes = [ctypes.POINTER(i64), i32, ctypes.POINTER(GpuParams), ctypes.POINTER(P)]
+    lib.sadcs_hip_ctx_create_batch.restype = ctypes.c_int
     lib.sadcs_hip_device_alloc.argtypes = [ctypes.c_size_t]
     lib.sadcs_hip_device_alloc.restype = P
     lib.sadcs_hip_device_free.argtypes = [P]
@@ -126,7 +128,7 @@ EXPORTED_SYMBOLS = (
     "sadcs_hip_alloc_pinned", "sadcs_hip_free_pinned", "sadcs_hip_last_error", "sadcs_hip_batch_build",
     "sadcs_hip_batch_required_bytes", "sadcs_hip_batch_layout", "sadcs_hip_batch_build_device",
     "sadcs_hip_server_create", "sadcs_hip_server_submit", "sadcs_hip_server_wait", "sadcs_hip_server_counters",
-    "sadcs_hip_server_destroy", "sadcs_hip_device_alloc", "sadcs_hip_device_free", "sadcs_hip_memcpy",
+    "sadcs_hip_server_destroy", "sadcs_hip_ctx_create_batch", "sadcs_hip_device_alloc", "sadcs_hip_device_free", "sadcs_hip_memcpy",
     "sadcs_hip_memset", "sadcs_hip_synchronize",
 )
 
@@ -169,10 +171,18 @@ def _as_bytes(text):
 class GpuContext:
     """gpusa.h:84-95: device memory for inputs up to capacity bytes, kept across calls."""
 
-    def __init__(self, capacity, cover=32, splitter_spacing=0, tile_splitters=0):
+    def __init__(self, capacity, cover=32, splitter_spacing=0, tile_splitters=0, batch_lengths=None):
+        """capacity: the longest single text; batch_lengths: plan for a batch of these lengths instead
+        (sadcs_hip_ctx_create_batch)."""
         params = GpuParams(cover, splitter_spacing, tile_splitters)
         handle = ctypes.c_void_p()
-        _check(lib.sadcs_hip_ctx_create(int(capacity), ctypes.byref(params), ctypes.byref(handle)))
+        if batch_lengths is not None:
+            lengths = np.ascontiguousarray(batch_lengths, dtype=np.int64)
+            _check(lib.sadcs_hip_ctx_create_batch(lengths.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), lengths.size,
+                                                  ctypes.byref(params), ctypes.byref(handle)))
+            capacity = int(lengths.max()) if lengths.size else 0
+        else:
+            _check(lib.sadcs_hip_ctx_create(int(capacity), ctypes.byref(params), ctypes.byref(handle)))
         self.handle = handle
         self.capacity = capacity
         self.cover = cover

@@ -97,6 +97,40 @@ struct ClassWalkSource {
     }
 };
 
+// A walk evaluated once into arrays (walk order; value ~0 marks a dropped element), so that a
+// class whose characters come from the text pays for its random text reads once, not once in the
+// histogram pass and again in the scatter pass. Only the four characters are kept: a batch's text
+// index (the key's high word) follows from the position again.
+template <class Source, class K>
+__global__ void materializeWalkKernel(Source source, uint32_t count, uint32_t* __restrict__ chars, uint32_t* __restrict__ values) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        K key = 0;
+        uint32_t value = 0;
+        const bool kept = source(i, key, value);
+        chars[i] = static_cast<uint32_t>(key);
+        values[i] = kept ? value : ~0u;
+    }
+}
+
+template <class K, class Texts>
+struct MaterializedWalk {
+    const uint32_t* chars;
+    const uint32_t* values;
+    Texts texts;
+
+    __device__ __forceinline__ bool operator()(uint32_t i, K& key, uint32_t& value) const {
+        value = values[i];
+        key = chars[i];
+        if constexpr (sizeof(K) == 8) {
+            if (value != ~0u) {
+                key |= static_cast<K>(texts.segmentOf(value)) << 32;
+            }
+        }
+        return value != ~0u;
+    }
+};
+
 // Class 0 of a batch needs no work: phase (a) ranks by (text, suffix), and the sorted sample
 // array is indexed by those global ranks.
 

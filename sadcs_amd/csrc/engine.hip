@@ -46,19 +46,42 @@ __device__ __forceinline__ const CoverTables<128>& deviceCover<128>() {
     return gCover128;
 }
 
+// Cover 32 answers residue questions by bit arithmetic on the cover mask (no table load, which
+// would add a dependent memory round trip in front of the gathers that use the answer).
 template <int V>
-__device__ __forceinline__ uint32_t packPosition(uint32_t p) {
-    return (p >> DifferenceCover<V>::kLog2) * CoverSet<V>::kSize + deviceCover<V>().below[p & (V - 1)];
+__device__ __forceinline__ uint32_t coverBelow(uint32_t r) {
+    if constexpr (V == 32) {
+        return __builtin_popcount(coverMask<32>() & ((1u << r) - 1));
+    } else {
+        return deviceCover<V>().below[r];
+    }
 }
 
 template <int V>
+__device__ __forceinline__ uint32_t packPosition(uint32_t p) {
+    return (p >> DifferenceCover<V>::kLog2) * CoverSet<V>::kSize + coverBelow<V>(p & (V - 1));
+}
+
+// Position of compact index t. The residue comes from a chain of selects over the cover's
+// elements (compile-time constants), not a table in memory: a load here would stall each store.
+template <int V>
 __device__ __forceinline__ uint32_t unpackIndex(uint32_t t) {
-    return (t / CoverSet<V>::kSize) * V + deviceCover<V>().residue[t % CoverSet<V>::kSize];
+    const uint32_t e = t % CoverSet<V>::kSize;
+    uint32_t residue = 0;
+#pragma unroll
+    for (int k = 1; k < CoverSet<V>::kSize; k++) {
+        residue = e == static_cast<uint32_t>(k) ? static_cast<uint32_t>(CoverSet<V>::kElements[k]) : residue;
+    }
+    return (t / CoverSet<V>::kSize) * V + residue;
 }
 
 template <int V>
 __device__ __forceinline__ uint32_t classOf(uint32_t p) {
-    return deviceCover<V>().distance[p & (V - 1)];
+    if constexpr (V == 32) {
+        return __builtin_ctz(windowOf32(coverMask<32>(), p));
+    } else {
+        return deviceCover<V>().distance[p & (V - 1)];
+    }
 }
 
 // ==== small kernels ==============================================================================
@@ -382,6 +405,11 @@ uint64_t classSizeOf(uint64_t n, int d) {
     return size;
 }
 
+// The first ranking pass of a large sample leaves its resolved rank stores to
+// scatterResolvedRanksKernel, in this many launches over consecutive rank ranges.
+constexpr uint32_t kDeferredRankMin = 1u << 22;
+constexpr uint32_t kDeferredRankRounds = 8;
+
 template <int V>
 uint32_t mergeCapacity(uint32_t g, uint32_t k) {
     return k * g + DifferenceCover<V>::kClasses * (g - 1);
@@ -495,6 +523,7 @@ class Pipeline {
         SADCS_TRY(m_arena.allocate(m_lists[0], std::max<uint32_t>(m_samples, 1)));
         m_listLength[0] = m_samples;
         fillWords(m_ranks, rankWords, 0);
+        fillWords(m_lists[0], std::max<uint32_t>(m_samples, 1), ~0u);  // unwritten sorted-sample entries
 
         const uint32_t M = m_samples;
         const uint32_t groupCapacity = M / 2 + m_segments + 1;
@@ -589,7 +618,9 @@ class Pipeline {
                 spec.doubling = false;
                 spec.offset = static_cast<uint32_t>(depth);
                 spec.charBits = sigma == 256 ? 9 : charBits;
-                spec.chars = std::max(1, (64 - ordinalBits) / spec.charBits);
+                // The first pass sorts every sample: 56 key bits (7 digit passes) instead of up to 63
+                // (8) resolves nearly as many, and the later passes sort only the few left.
+                spec.chars = std::max(1, (first ? std::min(56, 64 - ordinalBits) : 64 - ordinalBits) / spec.charBits);
                 spec.subkeyBits = spec.chars * spec.charBits;
             } else {
                 const uint64_t h = depth / V * V;
@@ -628,8 +659,18 @@ class Pipeline {
             ms.nextGroupIds = groupIds[1 - gcur];
             ms.counts = counts;
             ms.samples = M;
+            ms.deferResolved = first && u >= kDeferredRankMin;
             scanPipeline<MarkVec, kMarkItems>(m_stream, u, MarkProducer{ms}, MarkConsumer<V>{ms}, markAggregates);
             SADCS_HIP(hipGetLastError());
+            if (ms.deferResolved) {
+                const uint32_t rankCount = static_cast<uint32_t>(Cover::pack(m_job.span));
+                const uint32_t step = (rankCount + kDeferredRankRounds - 1) / kDeferredRankRounds;
+                const unsigned grid = std::min<unsigned>(blocksFor(M), 8192);
+                for (uint32_t lo = 0; lo < rankCount; lo += step) {
+                    scatterResolvedRanksKernel<V><<<grid, 256, 0, m_stream>>>(m_lists[0], M, lo, std::min(rankCount, lo + step), m_ranks);
+                }
+                SADCS_HIP(hipGetLastError());
+            }
             SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords, counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
             SADCS_HIP(hipStreamSynchronize(m_stream));
 
@@ -670,9 +711,19 @@ class Pipeline {
         uint32_t* tmpValues = nullptr;
         uint32_t* hist = nullptr;
         Sum1* aggregates = nullptr;
+        uint32_t* walkKeys = nullptr;  // the walks of classes that read the text, evaluated once
+        uint32_t* walkValues = nullptr;
         const uint64_t walk = largest + m_segments + static_cast<uint64_t>(m_listLength[0]);
+        uint64_t textWalk = 0;
+        for (int d = 1; d < kClasses; d++) {
+            if ((d - 1) % 4 == 0) {
+                textWalk = std::max<uint64_t>(textWalk, m_listLength[d - 1] + m_segments);
+            }
+        }
         int status = SADCS_OK;
         auto cleanup = [&]() {
+            m_arena.release(walkValues);
+            m_arena.release(walkKeys);
             m_arena.release(aggregates);
             m_arena.release(hist);
             m_arena.release(tmpValues);
@@ -686,6 +737,8 @@ class Pipeline {
         if (kBatch && status == SADCS_OK) status = m_arena.allocate(tmpValues, std::max<uint64_t>(largest, 1));
         if (status == SADCS_OK) status = m_arena.allocate(hist, radixHistWords(walk));
         if (status == SADCS_OK) status = m_arena.allocate(aggregates, radixAggregateBytes(walk) / sizeof(Sum1) + 1);
+        if (status == SADCS_OK) status = m_arena.allocate(walkKeys, std::max<uint64_t>(textWalk, 1));
+        if (status == SADCS_OK) status = m_arena.allocate(walkValues, std::max<uint64_t>(textWalk, 1));
         if (status != SADCS_OK || m_dry) {
             cleanup();
             return status;
@@ -722,7 +775,12 @@ class Pipeline {
             sb.keys[1] = keys[next];
             sb.values[1] = m_lists[d];
             sb.current = passes % 2 == 1 ? 0 : 1;
-            radixPass<Key>(m_stream, sb, source, count, 24, 8, static_cast<uint32_t>(m_listLength[d]));
+            if (fromText) {
+                materializeWalkKernel<decltype(source), Key><<<blocksFor(count), 256, 0, m_stream>>>(source, count, walkKeys, walkValues);
+                radixPass<Key>(m_stream, sb, MaterializedWalk<Key, Texts>{walkKeys, walkValues, m_texts}, count, 24, 8, static_cast<uint32_t>(m_listLength[d]));
+            } else {
+                radixPass<Key>(m_stream, sb, source, count, 24, 8, static_cast<uint32_t>(m_listLength[d]));
+            }
             for (int e = 0; e < extraPasses; e++) {
                 const int shift = 32 + e * kRadixBits;
                 const int width = std::min(kRadixBits, 32 + segmentBits - shift);
@@ -1521,7 +1579,8 @@ bool validParams(const Params& p) {
 int buildOn(sadcs_ctx* context, const Params& params, const Layout& layout, const BuildRequest& rq, sadcs_stats* stats) {
     if (context) {
         const bool deviceCall = rq.deviceText != nullptr;
-        const size_t need = planFor(*context, layout, rq.mode, !deviceCall, true);
+        // A device call brings its text and its outputs; a host call has the library own both.
+        const size_t need = planFor(*context, layout, rq.mode, !deviceCall, !deviceCall);
         if (need > context->arenaBytes) {
             setError("input needs %zu bytes of the context's %zu", need, context->arenaBytes);
             return SADCS_NOT_ENOUGH_MEMORY;
@@ -1531,7 +1590,7 @@ int buildOn(sadcs_ctx* context, const Params& params, const Layout& layout, cons
     const auto start = std::chrono::steady_clock::now();
     sadcs_ctx planner;
     planner.params = params;
-    const size_t need = planFor(planner, layout, rq.mode, rq.deviceText == nullptr, true);
+    const size_t need = planFor(planner, layout, rq.mode, rq.deviceText == nullptr, rq.deviceText == nullptr);
     if (need == SIZE_MAX) {
         setError("planning the device memory failed");
         return SADCS_FAILED;
@@ -1575,6 +1634,47 @@ int64_t sadcs_hip_required_bytes(int32_t n) {
     sadcs_ctx planner;
     const size_t plan = contextPlan(planner, n);
     return plan == SIZE_MAX ? -1 : static_cast<int64_t>(plan);
+}
+
+int sadcs_hip_ctx_create_batch(const int64_t* lengths, int32_t count, const sadcs_params* params, sadcs_ctx** context) {
+    tLastError[0] = '\0';
+    if (!context || (!lengths && count > 0) || count < 0) {
+        setError("invalid arguments");
+        return SADCS_FAILED;
+    }
+    *context = nullptr;
+    Params p;
+    p.apply(params);
+    if (!validParams(p)) {
+        return SADCS_FAILED;
+    }
+    int64_t total = 0, longest = 0;
+    for (int32_t i = 0; i < count; i++) {
+        if (lengths[i] < 1 || lengths[i] > INT32_MAX) {
+            setError("text %d has length %lld", i, static_cast<long long>(lengths[i]));
+            return SADCS_FAILED;
+        }
+        total += lengths[i];
+        longest = std::max(longest, lengths[i]);
+    }
+    if (total + int64_t(count) * kSegmentAlign > INT32_MAX) {
+        setError("a batch holds at most 2^31 - 1 positions");
+        return SADCS_NOT_ENOUGH_MEMORY;
+    }
+    sadcs_ctx planner;
+    planner.params = p;
+    size_t plan = contextPlan(planner, longest);
+    if (count > 0) {
+        const Layout layout = layoutFor(p, lengths, count, true);
+        for (Mode mode : {Mode::SA, Mode::BWT, Mode::Both}) {
+            plan = std::max(plan, planFor(planner, layout, mode, true, true));
+        }
+    }
+    if (plan == SIZE_MAX) {
+        setError("planning the device memory failed");
+        return SADCS_FAILED;
+    }
+    return createContext(plan, p, static_cast<int32_t>(longest), *context);
 }
 
 int sadcs_hip_ctx_create(int32_t capacity, const sadcs_params* params, sadcs_ctx** context) {

@@ -65,6 +65,9 @@ __device__ __forceinline__ const CoverTables<V>& deviceCover();
 template <int V>
 __device__ __forceinline__ uint32_t packPosition(uint32_t p);
 
+template <int V>
+__device__ __forceinline__ uint32_t coverBelow(uint32_t r);
+
 struct MergeCtx {
     const uint64_t* text;
     const uint32_t* ranks;
@@ -246,7 +249,7 @@ __device__ __forceinline__ void decodeRecord(const MergeCtx& c, const Texts& tex
 
 template <int V, class Texts>
 __device__ __forceinline__ void loadRecord(const MergeCtx& c, const Texts& texts, uint32_t p, MergeRecord<V, Texts::kBatch>& r, uint8_t* before = nullptr) {
-    const uint32_t below = deviceCover<V>().below[p & (V - 1)];
+    const uint32_t below = coverBelow<V>(p & (V - 1));
     RawRecord<V> raw;
     fetchRecord<V>(c, p, below, raw);
     decodeRecord<V, Texts>(c, texts, p, below, raw, r, before);
@@ -442,26 +445,53 @@ __global__ void boundaryRowsKernel(MergeCtx c, Texts texts, ListSet lists, ListS
 
 // ---- tile layout ----------------------------------------------------------------------------
 
-// Reads tile `tile`'s rows: every list's part must lie inside the list, rows must not go backwards
-// and the parts must add up to the difference of the output bases. False (nothing to do, and an
-// internal error if flagged by the caller) otherwise.
-__device__ __forceinline__ bool tileLayout(const ListSet& lists, const uint32_t* __restrict__ rows, uint64_t tile, uint32_t* segStart, uint32_t* rowStart, uint32_t& outBase) {
-    const int lc = lists.count;
-    const uint32_t stride = lc + 1;
-    uint32_t sum = 0;
-    bool ok = true;
-    for (int l = 0; l < lc; l++) {
-        const uint32_t a = rows[tile * stride + l], b = rows[(tile + 1) * stride + l];
-        ok = ok && a <= b && b <= lists.length[l];
-        segStart[l] = sum;
-        rowStart[l] = a;
-        sum += ok ? b - a : 0;
+// A tile's layout is read by wave 0 of the workgroup, one list per lane: tileRows loads the row
+// entries (a = the tile's, b = the next tile's), tilePublish checks them - every list's part must
+// lie inside the list, rows must not go backwards, the parts must add up to the difference of the
+// output bases and fit `capacity` - and writes the run starts segStart[0..lc], the list rows
+// rowStart[0..lc) (if wanted), the output base and the verdict. The caller synchronises before
+// reading them. Both are no-ops outside wave 0.
+__device__ __forceinline__ void tileRows(const ListSet& lists, const uint32_t* __restrict__ rows, uint64_t tile, uint32_t& a, uint32_t& b) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t lc = static_cast<uint32_t>(lists.count);
+    a = 0;
+    b = 0;
+    if (lane <= lc) {
+        a = rows[tile * (lc + 1) + lane];
+        b = rows[(tile + 1) * (lc + 1) + lane];
     }
-    segStart[lc] = sum;
-    outBase = rows[tile * stride + lc];
-    const uint32_t nextBase = rows[(tile + 1) * stride + lc];
-    ok = ok && outBase <= nextBase && nextBase - outBase == sum && nextBase <= lists.indexBase[lc];
-    return ok;
+}
+
+__device__ __forceinline__ void tilePublish(const ListSet& lists, uint32_t a, uint32_t b, uint32_t capacity, uint32_t* segStart, uint32_t* rowStart, uint32_t* outBase, uint32_t* valid) {
+    if (threadIdx.x >= kWave) {
+        return;
+    }
+    const uint32_t lane = threadIdx.x;
+    const uint32_t lc = static_cast<uint32_t>(lists.count);
+    const bool isList = lane < lc;
+    const bool ok = !isList || (a <= b && b <= lists.length[lane]);
+    const uint32_t length = isList && ok ? b - a : 0;
+    const uint32_t inclusive = waveInclusive<OpSum>(length);
+    const uint32_t base = readLane(a, lc), end = readLane(b, lc), total = readLane(inclusive, lc);
+    const bool allOk = __all(ok);
+    if (lane <= lc) {
+        segStart[lane] = inclusive - length;
+    }
+    if (isList && rowStart) {
+        rowStart[lane] = a;
+    }
+    if (lane == 0) {
+        *outBase = base;
+        *valid = allOk && base <= end && end - base == total && end <= lists.indexBase[lc] && total <= capacity;
+    }
+}
+
+__device__ __forceinline__ void tileLayout(const ListSet& lists, const uint32_t* __restrict__ rows, uint64_t tile, uint32_t capacity, uint32_t* segStart, uint32_t* rowStart, uint32_t* outBase, uint32_t* valid) {
+    if (threadIdx.x < kWave) {
+        uint32_t a, b;
+        tileRows(lists, rows, tile, a, b);
+        tilePublish(lists, a, b, capacity, segStart, rowStart, outBase, valid);
+    }
 }
 
 // ---- tile-major copy ----------------------------------------------------------------------
@@ -470,25 +500,42 @@ __device__ __forceinline__ bool tileLayout(const ListSet& lists, const uint32_t*
 __global__ void tileMajorCopyKernel(ListSet lists, const uint32_t* __restrict__ rows, uint32_t* __restrict__ out) {
     __shared__ uint32_t segStart[kMaxLists + 1];
     __shared__ uint32_t rowStart[kMaxLists];
-    __shared__ uint32_t outBase;
+    __shared__ uint32_t outBase, valid;
+    __shared__ const uint32_t* listPtr[kMaxLists];  // indexed per lane: LDS, not kernel-argument loads
     const int lc = lists.count;
     const uint64_t tile = blockIdx.x;
 
-    __shared__ bool valid;
-    if (threadIdx.x == 0) {
-        valid = tileLayout(lists, rows, tile, segStart, rowStart, outBase);
+    if (threadIdx.x < static_cast<uint32_t>(lc)) {
+        listPtr[threadIdx.x] = lists.ptr[threadIdx.x];
     }
+    tileLayout(lists, rows, tile, ~0u, segStart, rowStart, &outBase, &valid);
     __syncthreads();
     if (!valid) {
         return;
     }
 
-    int l = 0;
-    for (uint32_t e = threadIdx.x; e < segStart[lc]; e += blockDim.x) {
-        while (l + 1 < lc && e >= segStart[l + 1]) {
-            l++;
+    // Up to kCopyItems elements per thread: all loads first, then the stores (which the compiler
+    // must assume could alias the lists, so interleaving would cost a round trip per element).
+    constexpr int kCopyItems = 8;
+    const uint32_t size = segStart[lc];
+    for (uint32_t e0 = threadIdx.x; e0 < size; e0 += kCopyItems * blockDim.x) {
+        uint32_t v[kCopyItems];
+        int l = 0;
+#pragma unroll
+        for (int j = 0; j < kCopyItems; j++) {
+            const uint32_t e = min(e0 + j * blockDim.x, size - 1);
+            while (l + 1 < lc && e >= segStart[l + 1]) {
+                l++;
+            }
+            v[j] = listPtr[l][rowStart[l] + (e - segStart[l])];
         }
-        out[outBase + e] = lists.ptr[l][rowStart[l] + (e - segStart[l])];
+#pragma unroll
+        for (int j = 0; j < kCopyItems; j++) {
+            const uint32_t e = e0 + j * blockDim.x;
+            if (e < size) {
+                out[outBase + e] = v[j];
+            }
+        }
     }
 }
 
@@ -802,25 +849,22 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
     __shared__ uint32_t bounds[kMaxLists + 1];
     __shared__ uint32_t segStart[kMaxLists + 1];
     __shared__ uint32_t rowStart[kMaxLists];
-    __shared__ uint32_t outBase;
-    __shared__ bool valid;
+    __shared__ uint32_t outBase, valid;
 
     const int lc = lists.count;
     const uint64_t tile = blockIdx.x;
     const TieTables tables = setupTables<V>();
 
-    if (threadIdx.x == 0) {
-        valid = tileLayout(lists, rows, tile, segStart, rowStart, outBase) && segStart[lc] <= capacity;
-        for (int l = 0; l <= lc; l++) {
-            bounds[l] = segStart[l];
-        }
-        if (!valid) {
-            atomicOr(errors, 1u);
-        }
-    }
+    tileLayout(lists, rows, tile, capacity, segStart, rowStart, &outBase, &valid);
     __syncthreads();
     if (!valid) {
+        if (threadIdx.x == 0) {
+            atomicOr(errors, 1u);
+        }
         return;
+    }
+    if (threadIdx.x <= static_cast<uint32_t>(lc)) {
+        bounds[threadIdx.x] = segStart[threadIdx.x];
     }
     const uint32_t size = segStart[lc];
 

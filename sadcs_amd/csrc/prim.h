@@ -31,6 +31,18 @@ inline uint32_t ceilDiv(uint64_t a, uint64_t b) {
 
 // ==== scan pipeline ==============================================================================
 
+// A compiler-only fence (no instruction, no wait): memory operations are not moved across it, so
+// loads written before it are all issued before those written after it.
+__device__ __forceinline__ void orderLoads() {
+    __asm__ volatile("" ::: "memory");
+}
+
+// An item index for loads that are issued for every item, in range or not: past the end it reads
+// the last item (or item 0 of an empty range, which every buffer here has room for).
+__device__ __forceinline__ uint32_t clampIndex(uint64_t i, uint32_t count) {
+    return i < count ? static_cast<uint32_t>(i) : (count ? count - 1 : 0);
+}
+
 constexpr int kScanThreads = 256;
 constexpr int kScanWaves = kScanThreads / kWave;
 
@@ -42,11 +54,19 @@ __global__ __launch_bounds__(kScanThreads) void scanReduceKernel(Producer produc
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanThreads * ITEMS + wave * ITEMS * kWave + lane;
     VecT sum = VecT::zero();
 
+    // Every item's loads first, in straight-line code (indices past the end read the last item and
+    // are dropped), so they are in flight together rather than one round trip per item.
+    VecT items[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const uint64_t i = base + static_cast<uint64_t>(j) * kWave;
+        items[j] = produce(clampIndex(i, count));
+    }
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         const uint64_t i = base + static_cast<uint64_t>(j) * kWave;
         if (i < count) {
-            sum = VecT::combine(sum, produce(static_cast<uint32_t>(i)));
+            sum = VecT::combine(sum, items[j]);
         }
     }
 
@@ -122,7 +142,14 @@ __global__ __launch_bounds__(kScanThreads) void scanDownsweepKernel(Producer pro
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         const uint64_t i = base + static_cast<uint64_t>(j) * kWave;
-        items[j] = i < count ? produce(static_cast<uint32_t>(i)) : VecT::zero();
+        items[j] = produce(clampIndex(i, count));
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const uint64_t i = base + static_cast<uint64_t>(j) * kWave;
+        if (i >= count) {
+            items[j] = VecT::zero();
+        }
         const VecT inclusive = VecT::waveInclusive(items[j]);
         exclusive[j] = VecT::combine(carry, VecT::shiftUp(inclusive));
         carry = VecT::combine(carry, VecT::lane(inclusive, kWave - 1));
@@ -138,12 +165,24 @@ __global__ __launch_bounds__(kScanThreads) void scanDownsweepKernel(Producer pro
         prefix = VecT::combine(prefix, waveTotals[w]);
     }
 
+    // The consumer's loads for every item (clamped indices), then its stores (valid items only):
+    // stores could alias later loads, so loads interleaved with them would run one at a time.
+    // (Two stages, for loads that depend on other loads of the same item.)
+    typename Consumer::Prepared prepared[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        prepared[j] = consume.prepare(clampIndex(base + static_cast<uint64_t>(j) * kWave, count));
+    }
+    orderLoads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        consume.complete(prepared[j]);
+    }
+    orderLoads();
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         const uint64_t i = base + static_cast<uint64_t>(j) * kWave;
-        if (i < count) {
-            consume(static_cast<uint32_t>(i), items[j], VecT::combine(prefix, exclusive[j]));
-        }
+        consume(clampIndex(i, count), prepared[j], items[j], VecT::combine(prefix, exclusive[j]), i < count);
     }
 }
 
@@ -177,8 +216,15 @@ struct LoadU32 {
 
 struct StoreExclusiveU32 {
     uint32_t* out;
-    __device__ void operator()(uint32_t i, const Sum1&, const Sum1& exclusive) const {
-        out[i] = exclusive.v[0];
+    struct Prepared {};
+    __device__ Prepared prepare(uint32_t) const {
+        return {};
+    }
+    __device__ void complete(Prepared&) const {}
+    __device__ void operator()(uint32_t i, const Prepared&, const Sum1&, const Sum1& exclusive, bool valid) const {
+        if (valid) {
+            out[i] = exclusive.v[0];
+        }
     }
 };
 
@@ -239,15 +285,21 @@ __global__ __launch_bounds__(kSortThreads) void radixUpsweepKernel(Source source
 
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile + wave * kSortItems * kWave + lane;
 
+    // Every item's loads first (straight-line, clamped indices), then the counting.
+    K keys[kSortItems];
+    bool valids[kSortItems];
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
         const uint64_t i = base + static_cast<uint64_t>(j) * kWave;
-        K key = 0;
         uint32_t value = 0;
-        const bool valid = i < count && source(static_cast<uint32_t>(i), key, value);
-        const uint32_t digit = digitOf(key, shift, mask);
-        const uint64_t peers = matchDigit<kRadixBits>(digit, valid);
-        if (valid && lane == static_cast<uint32_t>(__builtin_ctzll(peers))) {
+        keys[j] = 0;
+        valids[j] = source(clampIndex(i, count), keys[j], value) && i < count;
+    }
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        const uint32_t digit = digitOf(keys[j], shift, mask);
+        const uint64_t peers = matchDigit<kRadixBits>(digit, valids[j]);
+        if (valids[j] && lane == static_cast<uint32_t>(__builtin_ctzll(peers))) {
             counts[wave][digit] += static_cast<uint32_t>(__builtin_popcountll(peers));
         }
     }
@@ -290,13 +342,17 @@ __global__ __launch_bounds__(kSortThreads) void radixDownsweepKernel(Source sour
     uint32_t ranks[kSortItems];
     bool valids[kSortItems];
 
-    // Rank each item among the wave's earlier items with the same digit.
+    // Every item's loads first (straight-line, clamped indices).
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
         const uint64_t i = base + static_cast<uint64_t>(j) * kWave;
         keys[j] = 0;
         values[j] = 0;
-        valids[j] = i < count && source(static_cast<uint32_t>(i), keys[j], values[j]);
+        valids[j] = source(clampIndex(i, count), keys[j], values[j]) && i < count;
+    }
+    // Rank each item among the wave's earlier items with the same digit.
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
         const uint32_t digit = digitOf(keys[j], shift, mask);
         const uint64_t peers = matchDigit<kRadixBits>(digit, valids[j]);
         const uint32_t before = valids[j] ? waveCounts[wave][digit] : 0;

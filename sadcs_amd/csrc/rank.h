@@ -153,25 +153,26 @@ struct MarkState {
     uint32_t* nextGroupIds;
     uint32_t* counts;          // [0] unresolved left, [1] groups left
     uint32_t samples;          // entries of sortedSample
+    bool deferResolved;        // resolved elements' rank stores are left to scatterResolvedRanksKernel
 };
 
 struct MarkFlags {
     bool head, groupStart, resolved;
 };
 
+// Branch-free: the three loads of an item are issued together with those of the other items.
 __device__ __forceinline__ MarkFlags markFlags(const MarkState& s, uint32_t i) {
     const uint64_t key = s.keys[i];
+    const uint64_t previous = s.keys[i > 0 ? i - 1 : 0];
+    const uint64_t next = s.keys[i + 1 < s.u ? i + 1 : i];
+    // Bitwise operators: a short-circuit would let the compiler move a load under a branch.
+    const bool first = i == 0, last = i + 1 == s.u;
+    const int shift = s.subkeyBits < 64 ? s.subkeyBits : 0;
+    const uint64_t groupMask = s.subkeyBits < 64 ? ~uint64_t(0) : 0;
     MarkFlags f;
-    if (i == 0) {
-        f.head = true;
-        f.groupStart = true;
-    } else {
-        const uint64_t previous = s.keys[i - 1];
-        f.head = previous != key;
-        f.groupStart = s.subkeyBits >= 64 ? false : (previous >> s.subkeyBits) != (key >> s.subkeyBits);
-    }
-    const bool nextHead = i + 1 == s.u || s.keys[i + 1] != key;
-    f.resolved = f.head && nextHead;
+    f.head = first | (previous != key);
+    f.groupStart = first | ((((previous >> shift) ^ (key >> shift)) & groupMask) != 0);
+    f.resolved = f.head & (last | (next != key));
     return f;
 }
 
@@ -191,15 +192,30 @@ struct MarkProducer {
 template <int V>
 struct MarkConsumer {
     MarkState s;
-    __device__ void operator()(uint32_t i, const MarkVec& item, const MarkVec& exclusive) const {
+    struct Prepared {
+        uint64_t key;
+        uint32_t t, groupId;
+    };
+    // The loads of item i (any index in range): its key and value, then its group's id.
+    __device__ Prepared prepare(uint32_t i) const {
+        return Prepared{s.keys[i], s.values[i], 0};
+    }
+    __device__ void complete(Prepared& p) const {
+        p.groupId = s.groupIds[s.subkeyBits >= 64 ? 0 : static_cast<uint32_t>(p.key >> s.subkeyBits)];
+    }
+    // The stores of item i, for valid items only.
+    __device__ void operator()(uint32_t i, const Prepared& p, const MarkVec& item, const MarkVec& exclusive, bool valid) const {
         const uint32_t lastHead = max(exclusive.v[0], item.v[0]);
         const uint32_t groupStart = max(exclusive.v[1], item.v[1]);
-        const uint64_t key = s.keys[i];
-        const uint32_t ordinal = s.subkeyBits >= 64 ? 0 : static_cast<uint32_t>(key >> s.subkeyBits);
-        const uint32_t id = s.groupIds[ordinal] + (lastHead - groupStart);
-        const uint32_t t = s.values[i];
+        const uint32_t id = p.groupId + (lastHead - groupStart);
+        const uint32_t t = p.t;
+        if (!valid) {
+            return;
+        }
 
-        s.ranks[t] = id;
+        if (!(s.deferResolved && item.v[2] == 0)) {
+            s.ranks[t] = id;
+        }
         if (item.v[2] == 0) {
             if (id < s.samples) {
                 s.sortedSample[id] = unpackIndex<V>(t);
@@ -218,6 +234,25 @@ struct MarkConsumer {
         }
     }
 };
+
+// The deferred rank stores of one pass: ranks[pack(sortedSample[i])] = i for every entry written
+// in that pass (entries still holding ~0 are skipped). The mark scan meets resolved elements in
+// rank order, so their rank stores would scatter 4-byte writes over the whole rank array, each a
+// read-modify-write of a memory line; here one launch writes only the ranks in [first, last), a
+// range small enough for the partial lines to merge in the memory-side cache, and the launches
+// over consecutive ranges read the (coalesced) sorted sample again each time.
+template <int V>
+__global__ void scatterResolvedRanksKernel(const uint32_t* __restrict__ sortedSample, uint32_t samples, uint32_t first, uint32_t last, uint32_t* __restrict__ ranks) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < samples; i += gridDim.x * blockDim.x) {
+        const uint32_t p = sortedSample[i];
+        if (p != ~0u) {
+            const uint32_t t = packPosition<V>(p);
+            if (t >= first && t < last) {
+                ranks[t] = i;
+            }
+        }
+    }
+}
 
 __global__ void initGroupIdsKernel(const Segment* __restrict__ segments, uint32_t count, uint32_t* __restrict__ groupIds) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;

@@ -265,3 +265,19 @@ def test_device_single_equals_host(gpu):
     assert np.array_equal(d_sa.to_host(np.int32), want)
     assert (d_bwt.to_host(np.uint8).tobytes(), primary) == oracle.bwt_from_sa(text.tobytes(), want)
     assert stats.merge_kernel_seconds > 0 and stats.merge_kernel_elements == text.size
+
+
+def test_batch_many_texts(gpu):
+    """Enough texts that the text index takes most of a ranking key (12 bits): the key layout must
+    still fit 64 bits. Each text's outputs equal the oracle's."""
+    from sadcs_amd import workloads
+    rng = np.random.default_rng(12)
+    lengths = rng.integers(1, 300, 3000)
+    texts = [workloads.dna_like(int(n), 500 + i) if i % 3 else workloads.markov(int(n), 900 + i) for i, n in enumerate(lengths)]
+    texts[7] = np.frombuffer(b"a" * 250, dtype=np.uint8)  # one long run
+    results = gpu.gpu_batch_build(texts, True, True)
+    for i, (text, (sa, b, primary)) in enumerate(zip(texts, results)):
+        raw = text.tobytes()
+        want = oracle.naive_sa(raw)
+        assert np.array_equal(sa, want), f"text {i}"
+        assert (b, primary) == oracle.bwt_from_sa(raw, want), f"text {i}"
