@@ -9,7 +9,9 @@ entry point of the C ABI). value = input bytes per second over the timed steps, 
 Printed as one JSON line, with:
   roofline: the dominant kernel (the top-level tile merge) - its algorithmic bytes per launch
     (54 bytes per element for SA + BWT: list entry 4, record 44, SA 4, BWT 2) over its average
-    duration measured with HIP events on the library's stream, against 8 TB/s of HBM;
+    duration measured with HIP events on the library's stream, against 8 TB/s of HBM; traffic is
+    the kernel's measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy
+    of known size, profiles/mergeTilesKernel_c3_1GiB_traffic.md);
   e2e_hbm_fraction: the whole step's algorithmic bytes (127.94 bytes per input byte for fused
     SA + BWT) over the step time, against 8 TB/s;
   cpu_baseline: the reference's CPU construction (oracle/_ref, OpenMP on all host cores) timed
@@ -36,6 +38,18 @@ HBM_PEAK = 8.0e12
 BYTES_PER_INPUT_BYTE = 127.94   # fused SA + BWT, the reference design's logical traffic
 MERGE_BYTES_PER_ELEMENT = 54.0  # list entry 4 + record 44 + SA 4 + BWT 2
 METRIC = "SA+BWT input MB/s at 1 GiB; batched/served MB/s + p99 vs sequential"
+
+
+def measured_traffic(config, n):
+    """HBM bytes per launch of the top-level merge: the rocprofv3 FETCH_SIZE + WRITE_SIZE figure per
+    element, scaled by a copy of known size, from the committed summary for this configuration."""
+    path = os.path.join(ROOT, "profiles", f"mergeTilesKernel_{config}_1GiB_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_element:"):
+            return int(float(line.split(":")[1]) * n)
+    return None
 
 
 def cpu_baseline(text_prefix):
@@ -307,7 +321,7 @@ def main():
                    "cover": args.cover, "resident": "text in HBM before the timed region, outputs in HBM"},
         "roofline": {"bound": "hbm", "kernel": "mergeTilesKernel (top level, SA+BWT in place)",
                      "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic(args.config, n),
                      "alg_bytes_per_launch": int(MERGE_BYTES_PER_ELEMENT * n), "avg_launch_ms": round(kernel * 1e3, 3)},
         "e2e_hbm_fraction": round(e2e, 4),
         "phases_ms": {"ranking": round(ranking * 1e3, 2), "classes": round(classes * 1e3, 2), "merge": round(merge * 1e3, 2)},
