@@ -101,6 +101,11 @@ int sadcs_hip_build(sadcs_ctx* context, const uint8_t* text, int32_t n, int32_t*
 int sadcs_hip_build_device(sadcs_ctx* context, const void* d_text, int32_t n, void* d_sa, void* d_bwt,
                            int32_t* primary, sadcs_stats* stats);
 
+/* Device bytes the library itself needs for sadcs_hip_build_device on n bytes (cover 32): the
+ * caller's text and output buffers are not included, and before the merge writes them the
+ * outputs serve as scratch. No counterpart in the reference. */
+int64_t sadcs_hip_device_required_bytes(int32_t n, int32_t want_sa, int32_t want_bwt);
+
 /* gpusa.h:110-114 (gpuRankSample): uploads text and ranks the sample only;
  * ranks receives sadcs_hip_sample_size(n, 32) entries: ranks[pack(p)] for each sample position p. */
 int sadcs_hip_rank_sample(const uint8_t* text, int32_t n, uint32_t* ranks, sadcs_stats* stats);

@@ -104,6 +104,8 @@ def _load():
     lib.sadcs_hip_server_counters.restype = None
     lib.sadcs_hip_server_destroy.argtypes = [P]
     lib.sadcs_hip_server_destroy.restype = None
+    lib.sadcs_hip_device_required_bytes.argtypes = [i32, i32, i32]
+    lib.sadcs_hip_device_required_bytes.restype = i64
     lib.sadcs_hip_ctx_create_batch.argtypes = [ctypes.POINTER(i64), i32, ctypes.POINTER(GpuParams), ctypes.POINTER(P)]
     lib.sadcs_hip_ctx_create_batch.restype = ctypes.c_int
     lib.sadcs_hip_device_alloc.argtypes = [ctypes.c_size_t]
@@ -128,7 +130,7 @@ EXPORTED_SYMBOLS = (
     "sadcs_hip_alloc_pinned", "sadcs_hip_free_pinned", "sadcs_hip_last_error", "sadcs_hip_batch_build",
     "sadcs_hip_batch_required_bytes", "sadcs_hip_batch_layout", "sadcs_hip_batch_build_device",
     "sadcs_hip_server_create", "sadcs_hip_server_submit", "sadcs_hip_server_wait", "sadcs_hip_server_counters",
-    "sadcs_hip_server_destroy", "sadcs_hip_ctx_create_batch", "sadcs_hip_device_alloc", "sadcs_hip_device_free", "sadcs_hip_memcpy",
+    "sadcs_hip_server_destroy", "sadcs_hip_ctx_create_batch", "sadcs_hip_device_required_bytes", "sadcs_hip_device_alloc", "sadcs_hip_device_free", "sadcs_hip_memcpy",
     "sadcs_hip_memset", "sadcs_hip_synchronize",
 )
 

@@ -304,6 +304,62 @@ class Arena {
     size_t m_live = 0;
 };
 
+// Caller-owned device memory the pipeline may use as scratch until the merge writes it: the SA
+// buffer of a device call, and the BWT buffer of a fused one. A stack: the top moves down when its
+// last entry is released (entries may be released in any order). Allocations that do not fit go
+// to the arena instead. In a dry run the base is a placeholder that is never dereferenced.
+class ScratchStack {
+ public:
+    void reset(void* base, size_t size) {
+        m_base = static_cast<uint8_t*>(base);
+        m_size = base ? size : 0;
+        m_top = 0;
+        m_entries.clear();
+    }
+
+    template <class T>
+    bool take(T*& pointer, uint64_t count) {
+        const size_t bytes = (std::max<size_t>(count, 1) * sizeof(T) + Arena::kAlign - 1) / Arena::kAlign * Arena::kAlign;
+        if (!m_base || m_top + bytes > m_size) {
+            return false;
+        }
+        pointer = reinterpret_cast<T*>(m_base + m_top);
+        m_entries.push_back(Entry{m_top, bytes, false});
+        m_top += bytes;
+        return true;
+    }
+
+    // False if the pointer is not from this stack.
+    template <class T>
+    bool give(T*& pointer) {
+        const uint8_t* p = reinterpret_cast<const uint8_t*>(pointer);
+        if (!pointer || !m_base || p < m_base || p >= m_base + m_size) {
+            return false;
+        }
+        const size_t offset = static_cast<size_t>(p - m_base);
+        for (Entry& entry : m_entries) {
+            if (entry.offset == offset) {
+                entry.freed = true;
+            }
+        }
+        while (!m_entries.empty() && m_entries.back().freed) {
+            m_top = m_entries.back().offset;
+            m_entries.pop_back();
+        }
+        pointer = nullptr;
+        return true;
+    }
+
+ private:
+    struct Entry {
+        size_t offset, bytes;
+        bool freed;
+    };
+    uint8_t* m_base = nullptr;
+    size_t m_size = 0, m_top = 0;
+    std::vector<Entry> m_entries;
+};
+
 // ==== jobs =======================================================================================
 
 struct Params {
@@ -437,6 +493,10 @@ class Pipeline {
         m_ctxMerge.n = job.span;
         m_g = ctx.params.g;
         m_k = ctx.params.k ? ctx.params.k : defaultTileSplitters<V, kBatch>(m_g);
+        // Caller-owned outputs are idle until the merge writes them. The BWT buffer can hold the
+        // ranks only when the merge reads records from the lines (fused SA + BWT).
+        m_saScratch.reset(job.sa, job.sa ? static_cast<size_t>(job.totalLength) * sizeof(uint32_t) : 0);
+        m_bwtScratch.reset(job.sa && job.bwt && !job.rankOnly ? job.bwt : nullptr, job.bwt ? static_cast<size_t>(job.totalLength) : 0);
     }
 
     ~Pipeline() {
@@ -486,6 +546,23 @@ class Pipeline {
     uint32_t m_samples;
     MergeCtx m_ctxMerge{};
     uint32_t m_g = 0, m_k = 0;  // splitter spacing, sorted splitters per tile
+    ScratchStack m_saScratch;   // the caller's SA buffer before the merge
+    ScratchStack m_bwtScratch;  // the caller's BWT buffer before the merge (fused builds)
+
+    template <class T>
+    int scratch(T*& pointer, uint64_t count) {
+        if (m_saScratch.take(pointer, count)) {
+            return SADCS_OK;
+        }
+        return m_arena.allocate(pointer, count);
+    }
+
+    template <class T>
+    void unscratch(T*& pointer) {
+        if (!m_saScratch.give(pointer) && !m_bwtScratch.give(pointer)) {
+            m_arena.release(pointer);
+        }
+    }
 
     uint32_t* m_ranks = nullptr;
     uint32_t* m_lists[kClasses] = {};
@@ -502,7 +579,7 @@ class Pipeline {
         for (int d = 0; d < kClasses; d++) {
             m_arena.release(m_lists[d]);
         }
-        m_arena.release(m_ranks);
+        unscratch(m_ranks);
     }
 
     static unsigned blocksFor(uint64_t count, unsigned threads = 256) {
@@ -519,7 +596,9 @@ class Pipeline {
 
     int rankSample() {
         const uint64_t rankWords = static_cast<uint64_t>(Cover::pack(m_job.span)) + CoverSet<V>::kSize + 8;
-        SADCS_TRY(m_arena.allocate(m_ranks, rankWords));
+        if (!(V == kLinePositions && m_bwtScratch.take(m_ranks, rankWords))) {
+            SADCS_TRY(m_arena.allocate(m_ranks, rankWords));
+        }
         SADCS_TRY(m_arena.allocate(m_lists[0], std::max<uint32_t>(m_samples, 1)));
         m_listLength[0] = m_samples;
         fillWords(m_ranks, rankWords, 0);
@@ -546,12 +625,12 @@ class Pipeline {
             for (int i = 1; i >= 0; i--) {
                 m_arena.release(groupIds[i]);
                 m_arena.release(values[i]);
-                m_arena.release(keys[i]);
+                unscratch(keys[i]);
             }
         };
 
         for (int i = 0; i < 2 && status == SADCS_OK; i++) {
-            status = m_arena.allocate(keys[i], M);
+            status = scratch(keys[i], M);
             if (status == SADCS_OK) status = m_arena.allocate(values[i], M);
             if (status == SADCS_OK) status = m_arena.allocate(groupIds[i], groupCapacity);
         }
@@ -722,23 +801,23 @@ class Pipeline {
         }
         int status = SADCS_OK;
         auto cleanup = [&]() {
-            m_arena.release(walkValues);
-            m_arena.release(walkKeys);
+            unscratch(walkValues);
+            unscratch(walkKeys);
             m_arena.release(aggregates);
             m_arena.release(hist);
-            m_arena.release(tmpValues);
-            m_arena.release(tmpKeys);
-            m_arena.release(keys[1]);
-            m_arena.release(keys[0]);
+            unscratch(tmpValues);
+            unscratch(tmpKeys);
+            unscratch(keys[1]);
+            unscratch(keys[0]);
         };
-        status = m_arena.allocate(keys[0], std::max<uint64_t>(largest, 1));
-        if (status == SADCS_OK) status = m_arena.allocate(keys[1], std::max<uint64_t>(largest, 1));
-        if (kBatch && status == SADCS_OK) status = m_arena.allocate(tmpKeys, std::max<uint64_t>(largest, 1));
-        if (kBatch && status == SADCS_OK) status = m_arena.allocate(tmpValues, std::max<uint64_t>(largest, 1));
+        status = scratch(keys[0], std::max<uint64_t>(largest, 1));
+        if (status == SADCS_OK) status = scratch(keys[1], std::max<uint64_t>(largest, 1));
+        if (kBatch && status == SADCS_OK) status = scratch(tmpKeys, std::max<uint64_t>(largest, 1));
+        if (kBatch && status == SADCS_OK) status = scratch(tmpValues, std::max<uint64_t>(largest, 1));
         if (status == SADCS_OK) status = m_arena.allocate(hist, radixHistWords(walk));
         if (status == SADCS_OK) status = m_arena.allocate(aggregates, radixAggregateBytes(walk) / sizeof(Sum1) + 1);
-        if (status == SADCS_OK) status = m_arena.allocate(walkKeys, std::max<uint64_t>(textWalk, 1));
-        if (status == SADCS_OK) status = m_arena.allocate(walkValues, std::max<uint64_t>(textWalk, 1));
+        if (status == SADCS_OK) status = scratch(walkKeys, std::max<uint64_t>(textWalk, 1));
+        if (status == SADCS_OK) status = scratch(walkValues, std::max<uint64_t>(textWalk, 1));
         if (status != SADCS_OK || m_dry) {
             cleanup();
             return status;
@@ -850,8 +929,9 @@ class Pipeline {
                     reinterpret_cast<const uint4*>(m_job.text), m_ranks, count, m_lines);
                 SADCS_HIP(hipGetLastError());
             }
-            // Stream order keeps the build's reads of the ranks ahead of any reuse of their memory.
-            m_arena.release(m_ranks);
+            // Stream order keeps the build's reads of the ranks ahead of any reuse of their memory
+            // (the BWT writes, when the ranks were in the caller's BWT buffer).
+            unscratch(m_ranks);
             m_ctxMerge.ranks = nullptr;
             m_ctxMerge.lines = m_lines;
             return SADCS_OK;
@@ -893,10 +973,11 @@ class Pipeline {
         uint32_t* sorted = nullptr;
         uint32_t* counts = nullptr;
         Sum1* aggregates = nullptr;
-        int status = m_arena.allocate(positions, count);
-        if (status == SADCS_OK) status = m_arena.allocate(sorted, count);
-        if (status == SADCS_OK) status = m_arena.allocate(counts, static_cast<uint64_t>(boundaries) * lc);
-        if (status == SADCS_OK) status = m_arena.allocate(aggregates, scanScratchBytes<Sum1, kSumItems>(static_cast<uint64_t>(boundaries) * lc) / sizeof(Sum1) + 1);
+        // Scratch (the caller's SA buffer when there is one): released before the tiles run.
+        int status = scratch(positions, count);
+        if (status == SADCS_OK) status = scratch(sorted, count);
+        if (status == SADCS_OK) status = scratch(counts, static_cast<uint64_t>(boundaries) * lc);
+        if (status == SADCS_OK) status = scratch(aggregates, scanScratchBytes<Sum1, kSumItems>(static_cast<uint64_t>(boundaries) * lc) / sizeof(Sum1) + 1);
         for (int l = 0; l < lc && status == SADCS_OK; l++) {
             splitters.ptr[l] = positions + splitters.indexBase[l];
         }
@@ -919,10 +1000,10 @@ class Pipeline {
                 status = SADCS_FAILED;
             }
         }
-        m_arena.release(aggregates);
-        m_arena.release(counts);
-        m_arena.release(sorted);
-        m_arena.release(positions);
+        unscratch(aggregates);
+        unscratch(counts);
+        unscratch(sorted);
+        unscratch(positions);
         if (status != SADCS_OK) {
             m_arena.release(rows);
         }
@@ -1633,6 +1714,18 @@ int64_t sadcs_hip_required_bytes(int32_t n) {
     }
     sadcs_ctx planner;
     const size_t plan = contextPlan(planner, n);
+    return plan == SIZE_MAX ? -1 : static_cast<int64_t>(plan);
+}
+
+int64_t sadcs_hip_device_required_bytes(int32_t n, int32_t want_sa, int32_t want_bwt) {
+    if (n <= 0 || (!want_sa && !want_bwt)) {
+        return 0;
+    }
+    sadcs_ctx planner;
+    const int64_t lengths[1] = {n};
+    const Layout layout = makeLayout<32>(lengths, 1, false);
+    const Mode mode = want_sa && want_bwt ? Mode::Both : want_sa ? Mode::SA : Mode::BWT;
+    const size_t plan = planFor(planner, layout, mode, false, false);
     return plan == SIZE_MAX ? -1 : static_cast<int64_t>(plan);
 }
 

@@ -59,3 +59,12 @@ def test_no_device_is_unavailable():
     assert info.value.status == sadcs_amd.GpuStatus.Unavailable
     # n <= 1 needs no device work but still reports the device status, as gpusa.cu:528-542.
     assert sadcs_amd.gpu_build_status(b"", n=0) == sadcs_amd.GpuStatus.Unavailable
+
+
+def test_device_call_memory_plan():
+    """A device-resident fused build needs about 4n of its own beyond the caller's text (n + 64),
+    SA (4n) and BWT (n) buffers: the caller's outputs serve as scratch before the merge, so the
+    total stays near 10 bytes per input byte."""
+    for n in (1 << 20, 1 << 26, 1 << 30):
+        own = sadcs_amd.lib.sadcs_hip_device_required_bytes(n, 1, 1)
+        assert 0 < own and (own + 6 * n) / n <= 10.4, (n, own / n)
