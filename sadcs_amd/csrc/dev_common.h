@@ -32,6 +32,20 @@ __device__ __forceinline__ uint64_t prefixMask(uint32_t remaining) {
     return remaining >= 8 ? ~uint64_t(0) : ~uint64_t(0) << (64 - 8 * remaining);
 }
 
+// ---- XCD-aware tile order -------------------------------------------------------------------------
+
+// MI355X hands workgroups to its 8 XCDs round robin by index, and each XCD has its own L2. This
+// maps workgroup `block` of `tiles` to a tile so that each XCD takes one contiguous range of tiles:
+// neighbouring tiles, whose outputs share cache lines, then run on the same XCD at about the same
+// time and their partial lines merge in its L2 instead of in memory. A bijection on [0, tiles).
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t xcdTile(uint32_t block, uint32_t tiles) {
+    const uint32_t q = tiles / kXcds, r = tiles % kXcds;
+    const uint32_t xcd = block % kXcds, slot = block / kXcds;
+    return xcd * q + min(xcd, r) + slot;
+}
+
 // ---- lanes ------------------------------------------------------------------------------------
 
 __device__ __forceinline__ uint32_t laneId() {

@@ -283,7 +283,8 @@ __global__ __launch_bounds__(kSortThreads) void radixUpsweepKernel(Source source
     }
     __syncthreads();
 
-    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile + wave * kSortItems * kWave + lane;
+    const uint32_t tile = xcdTile(blockIdx.x, tiles);
+    const uint64_t base = static_cast<uint64_t>(tile) * kSortTile + wave * kSortItems * kWave + lane;
 
     // Every item's loads first (straight-line, clamped indices), then the counting.
     K keys[kSortItems];
@@ -310,7 +311,7 @@ __global__ __launch_bounds__(kSortThreads) void radixUpsweepKernel(Source source
         for (int w = 0; w < kSortWaves; w++) {
             total += counts[w][d];
         }
-        hist[static_cast<uint64_t>(d) * tiles + blockIdx.x] = total;
+        hist[static_cast<uint64_t>(d) * tiles + tile] = total;
     }
 }
 
@@ -331,12 +332,13 @@ __global__ __launch_bounds__(kSortThreads) void radixDownsweepKernel(Source sour
     for (uint32_t x = threadIdx.x; x < kSortWaves * kRadixBins; x += kSortThreads) {
         (&waveCounts[0][0])[x] = 0;
     }
+    const uint32_t tile = xcdTile(blockIdx.x, tiles);
     for (uint32_t d = threadIdx.x; d < kRadixBins; d += kSortThreads) {
-        globalStart[d] = offsets[static_cast<uint64_t>(d) * tiles + blockIdx.x];
+        globalStart[d] = offsets[static_cast<uint64_t>(d) * tiles + tile];
     }
     __syncthreads();
 
-    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile + wave * kSortItems * kWave + lane;
+    const uint64_t base = static_cast<uint64_t>(tile) * kSortTile + wave * kSortItems * kWave + lane;
     K keys[kSortItems];
     uint32_t values[kSortItems];
     uint32_t ranks[kSortItems];
