@@ -990,7 +990,7 @@ class Pipeline {
             status = mergeLevel<kOutIndex>(splitters, childOut, errors);
         }
         if (status == SADCS_OK && !m_dry) {
-            countBoundarySplittersKernel<<<blocksFor(boundaries), 256, 0, m_stream>>>(splitters, sorted, count, k, boundaries, counts);
+            countBoundarySplittersKernel<<<blocksFor(static_cast<uint64_t>(boundaries) * kWave), 256, 0, m_stream>>>(splitters, sorted, count, k, boundaries, counts);
             exclusiveSum(m_stream, counts, counts, boundaries * static_cast<uint32_t>(lc), aggregates);
             boundaryRowsKernel<V, Texts><<<blocksFor(boundaries + 1ull), 256, 0, m_stream>>>(
                 m_ctxMerge, m_texts, lists, splitters, sorted, counts, g, k, boundaries, rows);

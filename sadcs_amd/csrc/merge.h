@@ -359,23 +359,28 @@ __global__ void gatherSplittersKernel(ListSet lists, ListSet splitters, uint32_t
     }
 }
 
-// counts[l * boundaries + b] = splitters of list l among sorted splitters [b*K, (b+1)*K).
-__global__ void countBoundarySplittersKernel(ListSet splitters, const uint32_t* __restrict__ sorted, uint32_t total, uint32_t perTile, uint32_t boundaries, uint32_t* __restrict__ counts) {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+// counts[l * boundaries + b] = splitters of list l among sorted splitters [b*K, (b+1)*K). One
+// wave per boundary: the lanes classify 64 splitters at a time and one ballot per list counts them.
+__global__ __launch_bounds__(256) void countBoundarySplittersKernel(ListSet splitters, const uint32_t* __restrict__ sorted, uint32_t total, uint32_t perTile, uint32_t boundaries, uint32_t* __restrict__ counts) {
+    const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t lane = threadIdx.x % kWave;
     if (b >= boundaries) {
-        return;
+        return;  // whole waves leave together
     }
-    uint32_t local[kMaxLists];
-    for (int l = 0; l < splitters.count; l++) {
-        local[l] = 0;
-    }
+    const int lc = splitters.count;
     const uint64_t first = static_cast<uint64_t>(b) * perTile;
     const uint64_t last = min(first + perTile, static_cast<uint64_t>(total));
-    for (uint64_t x = first; x < last; x++) {
-        local[listOfIndex(splitters, sorted[x])]++;
+    uint32_t mine = 0;  // lane l < lc: the count of list l
+    for (uint64_t x0 = first; x0 < last; x0 += kWave) {
+        const uint64_t x = x0 + lane;
+        const int l = x < last ? listOfIndex(splitters, sorted[x]) : -1;
+        for (int k = 0; k < lc; k++) {
+            const uint32_t c = static_cast<uint32_t>(__builtin_popcountll(ballot(l == k)));
+            mine += lane == static_cast<uint32_t>(k) ? c : 0;
+        }
     }
-    for (int l = 0; l < splitters.count; l++) {
-        counts[static_cast<uint64_t>(l) * boundaries + b] = local[l];
+    if (lane < static_cast<uint32_t>(lc)) {
+        counts[static_cast<uint64_t>(lane) * boundaries + b] = mine;
     }
 }
 
