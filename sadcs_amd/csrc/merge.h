@@ -562,6 +562,7 @@ struct TileOut {
 };
 
 constexpr int kMergeThreads = 512;
+constexpr int kLoadSlots = 3;  // records each thread gathers at once (default tiles hold at most 1536)
 
 // A tile's records in LDS as a structure of arrays: a wave reading one field of 64 random records
 // then hits random banks. (As 64-byte structs, the same field of different records would map to
@@ -855,11 +856,15 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
     __shared__ uint32_t segStart[kMaxLists + 1];
     __shared__ uint32_t rowStart[kMaxLists];
     __shared__ uint32_t outBase, valid;
+    __shared__ const uint32_t* listPtr[kMaxLists];  // indexed per lane: LDS, not kernel-argument loads
 
     const int lc = lists.count;
     const uint64_t tile = blockIdx.x;
     const TieTables tables = setupTables<V>();
 
+    if (threadIdx.x < static_cast<uint32_t>(lc)) {
+        listPtr[threadIdx.x] = lists.ptr[threadIdx.x];
+    }
     tileLayout(lists, rows, tile, capacity, segStart, rowStart, &outBase, &valid);
     __syncthreads();
     if (!valid) {
@@ -873,40 +878,46 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
     }
     const uint32_t size = segStart[lc];
 
-    // Load the records, two per thread at a time so their gathers overlap; the BWT character before each.
+    // Load the records, kLoadSlots per thread at a time: every slot's position first, then every
+    // slot's record gathers, so a tile of up to kLoadSlots * kMergeThreads records costs two dependent
+    // round trips (straight-line code, clamped indices); the BWT character before each record too.
     {
-        int l = 0;
-        for (uint32_t e0 = threadIdx.x; e0 < size; e0 += 2 * kMergeThreads) {
-            Record r[2];
-            uint32_t pos[2];
+        for (uint32_t e0 = threadIdx.x; e0 < size; e0 += kLoadSlots * kMergeThreads) {
+            uint32_t pos[kLoadSlots];
+            int l = 0;
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const uint32_t e = e0 + h * kMergeThreads;
-                pos[h] = 0;
-                if (e < size) {
+            for (int h = 0; h < kLoadSlots; h++) {
+                const uint32_t e = min(e0 + h * kMergeThreads, size - 1);
+                if constexpr ((kFlags & kInPlace) != 0) {
+                    pos[h] = out.sa[outBase + e];
+                } else {
                     while (l + 1 < lc && e >= segStart[l + 1]) {
                         l++;
                     }
-                    uint32_t p = (kFlags & kInPlace) ? out.sa[outBase + e] : lists.ptr[l][rowStart[l] + (e - segStart[l])];
-                    if (p >= c.n) {
-                        atomicOr(errors, 2u);
-                        p = 0;
-                    }
-                    pos[h] = p;
+                    pos[h] = listPtr[l][rowStart[l] + (e - segStart[l])];
                 }
             }
-            uint8_t characterBefore[2];
+            RawRecord<V> raw[kLoadSlots];
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                loadRecord<V, Texts>(c, texts, pos[h], r[h], characterBefore + h);
+            for (int h = 0; h < kLoadSlots; h++) {
+                if (pos[h] >= c.n) {
+                    if (e0 + h * kMergeThreads < size) {
+                        atomicOr(errors, 2u);
+                    }
+                    pos[h] = 0;
+                }
+                fetchRecord<V>(c, pos[h], coverBelow<V>(pos[h] & (V - 1)), raw[h]);
             }
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < kLoadSlots; h++) {
                 const uint32_t e = e0 + h * kMergeThreads;
                 if (e < size) {
-                    recs.store(e, r[h]);
+                    Record r;
+                    uint8_t characterBefore;
+                    decodeRecord<V, Texts>(c, texts, pos[h], coverBelow<V>(pos[h] & (V - 1)), raw[h], r, &characterBefore);
+                    recs.store(e, r);
                     if constexpr ((kFlags & kOutBWT) != 0) {
-                        before[e] = characterBefore[h];
+                        before[e] = characterBefore;
                     }
                     indexA[e] = static_cast<uint16_t>(e);
                 }
