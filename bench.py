@@ -74,7 +74,7 @@ def cpu_baseline(text_prefix):
             "sample": f"first {n >> 20} MiB of the same text; reference CPU build (cover 128), SA {t1 - t0:.2f}s + BWT {t2 - t1:.2f}s"}
 
 
-def batched_measurement(count, steps):
+def batched_measurement(count, steps, tune=True):
     """Batched construction of `count` mixed texts against one-at-a-time device builds."""
     import numpy as np
 
@@ -108,8 +108,21 @@ def batched_measurement(count, steps):
     d_bwt = sadcs_amd.DeviceBuffer(total)
     d_sa2 = sadcs_amd.DeviceBuffer(4 * total)
     d_bwt2 = sadcs_amd.DeviceBuffer(total)
+    # Autotune on the first 256 texts (their own batch layout), then build everything with it.
+    tuned = {}
+    if tune:
+        sub_lengths = lengths[:256]
+        sub_offsets, sub_span = sadcs_amd.gpu_batch_layout(sub_lengths)
+        sub_host = np.zeros(sub_span + 64, dtype=np.uint8)
+        for t, o in zip(texts[:256], sub_offsets):
+            sub_host[o:o + t.size] = t
+        d_sub = sadcs_amd.DeviceBuffer.from_host(sub_host)
+        tuned, _ = sadcs_amd.gpu_tune(d_sub.ptr, sub_lengths)
+        d_sub.free()
+        print(f"# batched: tuned {tuned}", file=sys.stderr, flush=True)
     ctx = sadcs_amd.GpuContext(int(lengths.max()))
-    batch_ctx = sadcs_amd.GpuContext(0, batch_lengths=lengths)
+    batch_ctx = sadcs_amd.GpuContext(0, splitter_spacing=tuned.get("splitter_spacing", 0),
+                                     tile_splitters=tuned.get("tile_splitters", 0), batch_lengths=lengths)
     stats = sadcs_amd.GpuStats()
 
     batch_primaries = sadcs_amd.gpu_batch_build_device(batch_ctx, d_texts.ptr, lengths, d_sa.ptr, d_bwt.ptr)  # warmup
@@ -140,7 +153,8 @@ def batched_measurement(count, steps):
     return {"texts": count, "bytes": total, "mb_per_s": round(total / batch_seconds / 1e6, 1),
             "ms": round(batch_seconds * 1e3, 2), "sequential_mb_per_s": round(total / seq_seconds / 1e6, 1),
             "sequential_ms": round(seq_seconds * 1e3, 1), "speedup_vs_sequential": round(seq_seconds / batch_seconds, 2),
-            "outputs_equal_sequential": same, "resident": "texts in HBM, outputs in HBM"}
+            "outputs_equal_sequential": same, "resident": "texts in HBM, outputs in HBM",
+            "merge_params": tuned or "defaults"}
 
 
 def served_measurement(requests, factors=(0.5, 0.8, 1.1)):
@@ -214,6 +228,8 @@ def main():
     ap.add_argument("--cover", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-mib", type=int, default=256)
+    ap.add_argument("--tune-mib", type=int, default=64,
+                    help="autotune the merge configuration on a prefix of this many MiB before timing (0: defaults)")
     ap.add_argument("--batch-texts", type=int, default=4096, help="texts in the batched measurement (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=300, help="requests per offered rate (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -253,7 +269,17 @@ def main():
         target.free()
         print(f"# copied {args.copy_calibration} bytes device to device 3 times", file=sys.stderr, flush=True)
 
-    ctx = sadcs_amd.GpuContext(n, cover=args.cover)
+    tuned, tune_seconds = {}, 0.0
+    if args.tune_mib:
+        start = time.perf_counter()
+        sample = min(n, args.tune_mib << 20)
+        d_sample = sadcs_amd.DeviceBuffer.from_host(text[:sample], extra=64)
+        tuned, _ = sadcs_amd.gpu_tune(d_sample.ptr, [sample], cover=args.cover)
+        d_sample.free()
+        tune_seconds = time.perf_counter() - start
+        print(f"# tuned on {sample >> 20} MiB in {tune_seconds:.2f}s: {tuned}", file=sys.stderr, flush=True)
+    ctx = sadcs_amd.GpuContext(n, cover=args.cover, splitter_spacing=tuned.get("splitter_spacing", 0),
+                               tile_splitters=tuned.get("tile_splitters", 0))
     stats = sadcs_amd.GpuStats()
     for w in range(args.warmup):
         sadcs_amd.gpu_build_device(ctx, d_text.ptr, n, d_sa.ptr, d_bwt.ptr, stats)
@@ -301,7 +327,7 @@ def main():
     ctx.close()
     for buf in (d_text, d_sa, d_bwt):
         buf.free()
-    batched = batched_measurement(args.batch_texts, 3) if args.batch_texts else None
+    batched = batched_measurement(args.batch_texts, 3, tune=args.tune_mib > 0) if args.batch_texts else None
     served = served_measurement(args.serve_requests) if args.serve_requests else None
     ranking, classes, merge = (sum(p[i] for p in phases) / len(phases) for i in range(3))
     line = {
@@ -318,7 +344,8 @@ def main():
         "dtype": "u8 text, u32 positions/ranks, u64 sort keys",
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {description}", "n": n, "outputs": "SA int32 + BWT, one merge",
-                   "cover": args.cover, "resident": "text in HBM before the timed region, outputs in HBM"},
+                   "cover": args.cover, "resident": "text in HBM before the timed region, outputs in HBM",
+                   "merge_params": tuned or "defaults", "tuning_seconds": round(tune_seconds, 2)},
         "roofline": {"bound": "hbm", "kernel": "mergeTilesKernel (top level, SA+BWT in place)",
                      "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic(args.config, n),

@@ -95,9 +95,10 @@ int sadcs_hip_build(sadcs_ctx* context, const uint8_t* text, int32_t n, int32_t*
                     int32_t* primary, sadcs_stats* stats);
 
 /* The same with the text already on the device and the outputs left there (no host transfers).
- * d_text: n bytes followed by at least 64 zero bytes, 16-byte aligned. d_sa (4n bytes) and d_bwt
- * (n bytes) are device buffers or null. No counterpart in the reference: the device-resident
- * form of gpuSuffixArray/gpuBwt for callers that keep texts in HBM. */
+ * d_text: n bytes followed by at least 64 zero bytes, 16-byte aligned. d_sa (4n bytes, 4-byte
+ * aligned) and d_bwt (n bytes, any alignment) are device buffers or null; until the merge writes
+ * them the library uses them as scratch. No counterpart in the reference: the device-resident form
+ * of gpuSuffixArray/gpuBwt for callers that keep texts in HBM. */
 int sadcs_hip_build_device(sadcs_ctx* context, const void* d_text, int32_t n, void* d_sa, void* d_bwt,
                            int32_t* primary, sadcs_stats* stats);
 
@@ -141,6 +142,15 @@ int64_t sadcs_hip_batch_layout(const int64_t* lengths, int32_t count, int64_t* o
  * (d_sa: sum of lengths int32, local positions; d_bwt: sum of lengths bytes); primaries on the host. */
 int sadcs_hip_batch_build_device(sadcs_ctx* context, const void* d_texts, const int64_t* lengths, int32_t count,
                                  void* d_sa, void* d_bwt, int32_t* primaries, sadcs_stats* stats);
+
+/* ---- autotuner (no counterpart in the reference) ----
+ * Builds SA + BWT of a device-resident sample with each candidate merge configuration - splitter
+ * spacing G in {8, 12, 16, 24, 32} and tile splitters K at 1, 3/4 and 1/2 of the most that keep
+ * two tiles per CU - and writes the fastest (by device time, best of two builds) to *best; the
+ * cover comes from base (null: 32). count 1: d_texts is one text of lengths[0] bytes followed by
+ * 64 zero bytes; count > 1: a batch in the sadcs_hip_batch_layout layout. best_seconds may be null. */
+int sadcs_hip_tune(const void* d_texts, const int64_t* lengths, int32_t count, const sadcs_params* base,
+                   sadcs_params* best, double* best_seconds);
 
 /* ---- device memory for callers without their own HIP setup (no counterpart in the reference) ----
  * The library's HIP runtime allocates and copies, so a caller that keeps texts and outputs on the

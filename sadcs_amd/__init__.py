@@ -104,6 +104,9 @@ def _load():
     lib.sadcs_hip_server_counters.restype = None
     lib.sadcs_hip_server_destroy.argtypes = [P]
     lib.sadcs_hip_server_destroy.restype = None
+    lib.sadcs_hip_tune.argtypes = [P, ctypes.POINTER(i64), i32, ctypes.POINTER(GpuParams), ctypes.POINTER(GpuParams),
+                                   ctypes.POINTER(ctypes.c_double)]
+    lib.sadcs_hip_tune.restype = ctypes.c_int
     lib.sadcs_hip_device_required_bytes.argtypes = [i32, i32, i32]
     lib.sadcs_hip_device_required_bytes.restype = i64
     lib.sadcs_hip_ctx_create_batch.argtypes = [ctypes.POINTER(i64), i32, ctypes.POINTER(GpuParams), ctypes.POINTER(P)]
@@ -130,7 +133,7 @@ EXPORTED_SYMBOLS = (
     "sadcs_hip_alloc_pinned", "sadcs_hip_free_pinned", "sadcs_hip_last_error", "sadcs_hip_batch_build",
     "sadcs_hip_batch_required_bytes", "sadcs_hip_batch_layout", "sadcs_hip_batch_build_device",
     "sadcs_hip_server_create", "sadcs_hip_server_submit", "sadcs_hip_server_wait", "sadcs_hip_server_counters",
-    "sadcs_hip_server_destroy", "sadcs_hip_ctx_create_batch", "sadcs_hip_device_required_bytes", "sadcs_hip_device_alloc", "sadcs_hip_device_free", "sadcs_hip_memcpy",
+    "sadcs_hip_server_destroy", "sadcs_hip_ctx_create_batch", "sadcs_hip_device_required_bytes", "sadcs_hip_tune", "sadcs_hip_device_alloc", "sadcs_hip_device_free", "sadcs_hip_memcpy",
     "sadcs_hip_memset", "sadcs_hip_synchronize",
 )
 
@@ -442,3 +445,16 @@ def gpu_synchronize():
 
 def gpu_copy_device(dst, src, size):
     _check(lib.sadcs_hip_memcpy(dst, src, int(size), 3))
+
+
+def gpu_tune(d_texts, lengths, cover=32):
+    """sadcs_hip_tune: the fastest merge configuration for a device-resident sample (one text with
+    64 zero bytes after it, or a batch in the gpu_batch_layout layout). Returns
+    (dict(cover, splitter_spacing, tile_splitters), device seconds of the best build)."""
+    lengths = np.ascontiguousarray(np.atleast_1d(lengths), dtype=np.int64)
+    base = GpuParams(cover, 0, 0)
+    best = GpuParams()
+    seconds = ctypes.c_double(0)
+    _check(lib.sadcs_hip_tune(ctypes.c_void_p(d_texts), lengths.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), lengths.size,
+                              ctypes.byref(base), ctypes.byref(best), ctypes.byref(seconds)))
+    return {"cover": best.cover, "splitter_spacing": best.splitter_spacing, "tile_splitters": best.tile_splitters}, seconds.value

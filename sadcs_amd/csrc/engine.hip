@@ -308,11 +308,16 @@ class Arena {
 // buffer of a device call, and the BWT buffer of a fused one. A stack: the top moves down when its
 // last entry is released (entries may be released in any order). Allocations that do not fit go
 // to the arena instead. In a dry run the base is a placeholder that is never dereferenced.
+// The kernels assume the alignment of arena allocations, and a caller's buffer may start anywhere:
+// the base is rounded up to Arena::kAlign and the usable size is the same whatever the offset,
+// (size - (kAlign - 1)) rounded down to kAlign, so a dry run plans exactly what a build borrows.
 class ScratchStack {
  public:
     void reset(void* base, size_t size) {
-        m_base = static_cast<uint8_t*>(base);
-        m_size = base ? size : 0;
+        const uintptr_t b = reinterpret_cast<uintptr_t>(base);
+        const uintptr_t aligned = (b + Arena::kAlign - 1) / Arena::kAlign * Arena::kAlign;
+        m_size = base && size >= Arena::kAlign ? (size - (Arena::kAlign - 1)) / Arena::kAlign * Arena::kAlign : 0;
+        m_base = m_size ? reinterpret_cast<uint8_t*>(aligned) : nullptr;
         m_top = 0;
         m_entries.clear();
     }
@@ -1717,6 +1722,75 @@ int64_t sadcs_hip_required_bytes(int32_t n) {
     return plan == SIZE_MAX ? -1 : static_cast<int64_t>(plan);
 }
 
+int sadcs_hip_tune(const void* d_texts, const int64_t* lengths, int32_t count, const sadcs_params* base,
+                   sadcs_params* best, double* best_seconds) {
+    tLastError[0] = '\0';
+    if (!d_texts || !lengths || count < 1 || !best) {
+        setError("invalid arguments");
+        return SADCS_FAILED;
+    }
+    int64_t total = 0;
+    for (int32_t i = 0; i < count; i++) {
+        total += lengths[i];
+    }
+    if (total < 2 || total > INT32_MAX) {
+        setError("a tuning sample needs 2 to 2^31 - 1 bytes");
+        return SADCS_FAILED;
+    }
+    const int32_t cover = base && base->cover ? base->cover : 32;
+    const bool batch = count > 1;
+    void* d_sa = nullptr;
+    void* d_bwt = nullptr;
+    SADCS_HIP(hipMalloc(&d_sa, static_cast<size_t>(total) * sizeof(uint32_t)));
+    if (hipMalloc(&d_bwt, static_cast<size_t>(total)) != hipSuccess) {
+        (void)hipFree(d_sa);
+        setError("hipMalloc failed");
+        return SADCS_NOT_ENOUGH_MEMORY;
+    }
+    std::vector<int32_t> primaries(static_cast<size_t>(count));
+    double bestTime = 1e30;
+    int status = SADCS_OK;
+    for (uint32_t g : {8u, 12u, 16u, 24u, 32u}) {
+        const uint32_t most = cover == 128 ? (batch ? defaultTileSplitters<128, true>(g) : defaultTileSplitters<128, false>(g))
+                                           : (batch ? defaultTileSplitters<32, true>(g) : defaultTileSplitters<32, false>(g));
+        for (uint32_t k : {most, std::max(1u, 3 * most / 4), std::max(1u, most / 2)}) {
+            sadcs_params p{};
+            p.cover = cover;
+            p.splitter_spacing = g;
+            p.tile_splitters = k;
+            sadcs_ctx* ctx = nullptr;
+            status = sadcs_hip_ctx_create_batch(lengths, count, &p, &ctx);
+            if (status != SADCS_OK) {
+                break;
+            }
+            double fastest = 1e30;
+            for (int run = 0; run < 2 && status == SADCS_OK; run++) {
+                sadcs_stats stats{};
+                status = batch ? sadcs_hip_batch_build_device(ctx, d_texts, lengths, count, d_sa, d_bwt, primaries.data(), &stats)
+                               : sadcs_hip_build_device(ctx, d_texts, static_cast<int32_t>(lengths[0]), d_sa, d_bwt, primaries.data(), &stats);
+                fastest = std::min(fastest, stats.device_seconds);
+            }
+            sadcs_hip_ctx_destroy(ctx);
+            if (status != SADCS_OK) {
+                break;
+            }
+            if (fastest < bestTime) {
+                bestTime = fastest;
+                *best = p;
+            }
+        }
+        if (status != SADCS_OK) {
+            break;
+        }
+    }
+    (void)hipFree(d_bwt);
+    (void)hipFree(d_sa);
+    if (status == SADCS_OK && best_seconds) {
+        *best_seconds = bestTime;
+    }
+    return status;
+}
+
 int64_t sadcs_hip_device_required_bytes(int32_t n, int32_t want_sa, int32_t want_bwt) {
     if (n <= 0 || (!want_sa && !want_bwt)) {
         return 0;
@@ -1865,6 +1939,10 @@ int sadcs_hip_build_device(sadcs_ctx* context, const void* d_text, int32_t n, vo
     }
     if (!d_text || (reinterpret_cast<uintptr_t>(d_text) & 15) != 0) {
         setError("the device text must be 16-byte aligned");
+        return SADCS_FAILED;
+    }
+    if ((reinterpret_cast<uintptr_t>(d_sa) & 3) != 0) {
+        setError("the device suffix array must be 4-byte aligned");
         return SADCS_FAILED;
     }
     if (context && n > context->capacity) {
@@ -2031,6 +2109,10 @@ int sadcs_hip_batch_build_device(sadcs_ctx* context, const void* d_texts, const 
     }
     if (!d_texts || (reinterpret_cast<uintptr_t>(d_texts) & 15) != 0) {
         setError("the device texts must be 16-byte aligned");
+        return SADCS_FAILED;
+    }
+    if ((reinterpret_cast<uintptr_t>(d_sa) & 3) != 0) {
+        setError("the device suffix array must be 4-byte aligned");
         return SADCS_FAILED;
     }
     uint64_t total = 0;

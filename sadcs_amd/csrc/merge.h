@@ -780,7 +780,7 @@ __device__ __forceinline__ TieTables setupTables() {
 // every round, between barriers, on every thread.
 template <int V, class Texts, int kThreads, class AfterRound>
 __device__ __forceinline__ uint16_t* mergeRuns(const MergeCtx& c, const TieTables& tables, const LdsRecords<V, Texts::kBatch>& recs, uint16_t* src, uint16_t* dst, uint32_t* bounds, uint32_t runs, uint32_t size, AfterRound afterRound) {
-    const uint32_t chunk = max(4u, (size + kThreads - 1) / kThreads);
+    const uint32_t chunk = max(3u, (size + kThreads - 1) / kThreads);
     while (runs > 1) {
         for (uint64_t o = static_cast<uint64_t>(threadIdx.x) * chunk; o < size; o += static_cast<uint64_t>(kThreads) * chunk) {
             mergeChunk<V, Texts>(c, tables, recs, src, dst, bounds, runs, static_cast<uint32_t>(o), static_cast<uint32_t>(min(o + chunk, static_cast<uint64_t>(size))));

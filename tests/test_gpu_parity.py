@@ -281,3 +281,46 @@ def test_batch_many_texts(gpu):
         want = oracle.naive_sa(raw)
         assert np.array_equal(sa, want), f"text {i}"
         assert (b, primary) == oracle.bwt_from_sa(raw, want), f"text {i}"
+
+
+def test_tuner_picks_a_working_configuration(gpu):
+    """The autotuner returns a configuration whose builds equal the defaults' (single and batch)."""
+    from sadcs_amd import workloads
+    text = workloads.markov(2_000_000, 31)
+    d_text = gpu.DeviceBuffer.from_host(text, extra=64)
+    best, seconds = gpu.gpu_tune(d_text.ptr, [text.size])
+    assert best["cover"] == 32 and best["splitter_spacing"] in (8, 12, 16, 24, 32) and best["tile_splitters"] >= 1
+    assert seconds > 0
+    with gpu.GpuContext(text.size, splitter_spacing=best["splitter_spacing"], tile_splitters=best["tile_splitters"]) as ctx:
+        tuned = gpu.gpu_build(text, True, True, context=ctx)
+    plain = gpu.gpu_build(text, True, True)
+    assert np.array_equal(tuned[0], plain[0]) and tuned[1:] == plain[1:]
+    texts = [workloads.dna_like(n, 70 + i) for i, n in enumerate([5000, 12000, 700, 40000])]
+    lengths = np.array([t.size for t in texts], dtype=np.int64)
+    offsets, span = gpu.gpu_batch_layout(lengths)
+    host = np.zeros(span + 64, dtype=np.uint8)
+    for t, o in zip(texts, offsets):
+        host[o:o + t.size] = t
+    d_texts = gpu.DeviceBuffer.from_host(host)
+    best_batch, _ = gpu.gpu_tune(d_texts.ptr, lengths)
+    assert best_batch["splitter_spacing"] in (8, 12, 16, 24, 32)
+
+
+def test_device_outputs_at_any_offset(gpu):
+    """Device outputs may start at any byte offset of a larger buffer (as when many texts' outputs
+    are packed into one allocation); before the merge the library uses them as scratch, which it
+    must align itself. Offsets from a failure seen with packed outputs."""
+    from sadcs_amd import workloads
+    text = workloads.markov(151_701, 4 * 1_000_003 + 4)
+    n = text.size
+    want = gpu.gpu_build(text, True, True)
+    d_text = gpu.DeviceBuffer.from_host(text, extra=64)
+    for sa_offset, bwt_offset in [(12, 3), (8, 2), (4, 1), (0, 0)]:
+        d_sa = gpu.DeviceBuffer(4 * n + 1024)
+        d_bwt = gpu.DeviceBuffer(n + 1024)
+        with gpu.GpuContext(n) as ctx:
+            primary = gpu.gpu_build_device(ctx, d_text.ptr, n, d_sa.ptr + sa_offset, d_bwt.ptr + bwt_offset, gpu.GpuStats())
+        sa = d_sa.to_host(np.uint8)[sa_offset:sa_offset + 4 * n].view(np.int32)
+        bwt = d_bwt.to_host(np.uint8)[bwt_offset:bwt_offset + n].tobytes()
+        assert np.array_equal(sa, want[0]), f"offsets {sa_offset}, {bwt_offset}: SA"
+        assert (bwt, primary) == (want[1], want[2]), f"offsets {sa_offset}, {bwt_offset}: BWT"
