@@ -152,6 +152,26 @@ int sadcs_hip_batch_build_device(sadcs_ctx* context, const void* d_texts, const 
 int sadcs_hip_tune(const void* d_texts, const int64_t* lengths, int32_t count, const sadcs_params* base,
                    sadcs_params* best, double* best_seconds);
 
+/* ---- checking device-resident outputs (no counterpart in the reference's GPU API) ----
+ * The reference's fast suffix-array check (src/common/checker.cpp:53-109) on the device: every
+ * entry in [0, n), no duplicates, neighbours in order by first character and then by the rank of
+ * the following suffix. With d_bwt (may be null), the BWT and primary are checked against the
+ * suffix array in divsufsort's convention; a valid suffix array fixes them, so this passes exactly
+ * when the reference's inversion check (src/common/checker.cpp:196-245) would. result[0] is the
+ * first failing check in that order (SADCS_CHECK_*), result[1] the first failing index of the
+ * suffix array (range, duplicate, order: the index the reference reports) or of the BWT, else -1.
+ * Allocates 4n bytes of device memory for the inverse; synchronous. */
+enum {
+    SADCS_CHECK_PASS = 0,
+    SADCS_CHECK_RANGE = 1,      /* an entry outside [0, n) */
+    SADCS_CHECK_DUPLICATE = 2,  /* an entry that appears twice */
+    SADCS_CHECK_ORDER = 3,      /* SA[i] and SA[i + 1] out of order */
+    SADCS_CHECK_PRIMARY = 4,    /* primary differs from the row of suffix 0, plus one */
+    SADCS_CHECK_BWT = 5         /* a BWT byte differs */
+};
+int sadcs_hip_check_device(const void* d_text, int32_t n, const void* d_sa, const void* d_bwt, int32_t primary,
+                           int64_t* result);
+
 /* ---- device memory for callers without their own HIP setup (no counterpart in the reference) ----
  * The library's HIP runtime allocates and copies, so a caller that keeps texts and outputs on the
  * device needs no second runtime in the process. kind: 1 host to device, 2 device to host,

@@ -60,7 +60,7 @@ class GpuError(RuntimeError):
 
 def _load():
     if not os.path.exists(LIBRARY_PATH):
-        raise ImportError(f"{LIBRARY_PATH} is missing: build it with `python -m sadcs_amd.build`")
+        raise ImportError(f"{LIBRARY_PATH} is missing: build it with `python sadcs_amd/build.py`")
     lib = ctypes.CDLL(LIBRARY_PATH)
     P, i32, i64, u8p = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
     lib.sadcs_hip_query.argtypes = [ctypes.POINTER(GpuDevice)]
@@ -121,6 +121,8 @@ def _load():
     lib.sadcs_hip_memset.restype = ctypes.c_int
     lib.sadcs_hip_synchronize.argtypes = []
     lib.sadcs_hip_synchronize.restype = ctypes.c_int
+    lib.sadcs_hip_check_device.argtypes = [P, i32, P, P, i32, ctypes.POINTER(i64)]
+    lib.sadcs_hip_check_device.restype = ctypes.c_int
     return lib
 
 
@@ -134,7 +136,7 @@ EXPORTED_SYMBOLS = (
     "sadcs_hip_batch_required_bytes", "sadcs_hip_batch_layout", "sadcs_hip_batch_build_device",
     "sadcs_hip_server_create", "sadcs_hip_server_submit", "sadcs_hip_server_wait", "sadcs_hip_server_counters",
     "sadcs_hip_server_destroy", "sadcs_hip_ctx_create_batch", "sadcs_hip_device_required_bytes", "sadcs_hip_tune", "sadcs_hip_device_alloc", "sadcs_hip_device_free", "sadcs_hip_memcpy",
-    "sadcs_hip_memset", "sadcs_hip_synchronize",
+    "sadcs_hip_memset", "sadcs_hip_synchronize", "sadcs_hip_check_device",
 )
 
 
@@ -445,6 +447,21 @@ def gpu_synchronize():
 
 def gpu_copy_device(dst, src, size):
     _check(lib.sadcs_hip_memcpy(dst, src, int(size), 3))
+
+
+CHECK_NAMES = {0: "pass", 1: "entry out of range", 2: "duplicate entry", 3: "SA[index] and SA[index + 1] out of order",
+               4: "primary index differs", 5: "BWT byte differs"}
+
+
+def gpu_check_device(d_text, n, d_sa, d_bwt=None, primary=-1):
+    """sadcs_hip_check_device: checks a device-resident suffix array (and BWT with its primary)
+    on the GPU, as the reference's fast checker does (src/common/checker.cpp:53-109).
+    Returns (category, first failing index): (0, -1) on a pass; 1 out of range, 2 duplicate,
+    3 out of order (the index the reference reports), 4 primary, 5 BWT byte (its index)."""
+    result = (ctypes.c_int64 * 2)()
+    _check(lib.sadcs_hip_check_device(ctypes.c_void_p(d_text), int(n), ctypes.c_void_p(d_sa),
+                                      ctypes.c_void_p(d_bwt) if d_bwt else None, int(primary), result))
+    return int(result[0]), int(result[1])
 
 
 def gpu_tune(d_texts, lengths, cover=32):

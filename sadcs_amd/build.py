@@ -1,6 +1,7 @@
 """Builds the HIP library and the command-line tool in-tree for gfx950.
 
-    python -m sadcs_amd.build          # or: python sadcs_amd/build.py
+    python sadcs_amd/build.py          # (python -m sadcs_amd.build imports the package, which needs a
+                                       #  library matching its symbols)
 
 Outputs sadcs_amd/libsadcs_hip.so (the C ABI of include/sadcs_hip.h) and sadcs_amd/bin/sadcs
 (the command line). Every HIP compile disables packed FP32 instructions.

@@ -4,6 +4,7 @@
 #include "checker.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <vector>
 
@@ -44,8 +45,12 @@ namespace {
             if (p < 0 || p >= n) {
                 bad = std::min<int64_t>(bad, i);
             } else {
-#pragma omp atomic write
-                inverse[p] = i;
+                // The last of equal entries owns the slot, as in a sequential pass, so the first
+                // duplicate reported does not depend on the thread schedule.
+                std::atomic_ref<int32_t> slot(inverse[p]);
+                int32_t seen = slot.load(std::memory_order_relaxed);
+                while (seen < i && !slot.compare_exchange_weak(seen, i, std::memory_order_relaxed)) {
+                }
             }
         }
         if (bad != kNone) {

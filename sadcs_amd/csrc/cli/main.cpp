@@ -3,6 +3,10 @@
  * commands build|-b, check|-c, test|-t, bench, selftest with --bwt, --gpu, --cover, --threads,
  * --runs, --context, --pinned, --recursive, --lowmem, --naive, --verbose).
  *
+ * --device (check and test, suffix arrays) runs the fast check on the GPU instead of the host
+ * (sadcs_hip_check_device: the same checks, the same first failing index); it has no counterpart
+ * in the reference.
+ *
  * Every construction runs on the GPU: --gpu is accepted and implied (the reference's CPU variant
  * is not part of this build). --cover selects the GPU's difference cover, 32 (default) or 128.
  * --both builds the suffix array and the BWT in one merge and writes <out> and <out>.bwt.
@@ -23,6 +27,7 @@
 #include "../checker.h"
 #include "../fileio.h"
 #include "gpusa.h"
+#include "sadcs_hip.h"
 #include "selftest.h"
 
 namespace {
@@ -42,6 +47,7 @@ namespace {
         bool recursive = false;
         bool context = false;
         bool pinned = false;
+        bool deviceCheck = false;
         int runs = 5;
     };
 
@@ -53,15 +59,16 @@ namespace {
             "  sadcs build|-b <in> <out.sa> [--cover 32|128] [--gpu]\n"
             "  sadcs build|-b --bwt <in> <out.bwt> [--cover 32|128] [--gpu]\n"
             "  sadcs build|-b --both <in> <out.sa>      (also writes <out.sa>.bwt, one merge)\n"
-            "  sadcs check|-c <in> <in.sa> [--lowmem | --naive] [--threads N]\n"
+            "  sadcs check|-c <in> <in.sa> [--lowmem | --naive | --device] [--threads N]\n"
             "  sadcs check|-c --bwt <in> <in.bwt>\n"
-            "  sadcs test|-t <in|dir> [--bwt] [--cover 32|128] [--threads N] [--recursive]\n"
+            "  sadcs test|-t <in|dir> [--bwt] [--cover 32|128] [--threads N] [--recursive] [--device]\n"
             "  sadcs bench <in|dir> [--bwt | --both] [--runs N] [--context] [--pinned] [--recursive]\n"
             "  sadcs selftest [--verbose]\n"
             "\n"
             "The .sa file is n little-endian int32 entries (divsufsort's mksary format).\n"
             "The .bwt file is divsufsort's bwt format as one block: n, the primary index, the n bytes.\n"
             "test builds on the GPU and checks the result with the independent checker.\n"
+            "--device runs the suffix-array check on the GPU (text and array uploaded; 4n bytes of device memory).\n"
             "bench times the build host to host: a warmup, then the best of --runs, with each phase.\n"
             "--context reuses one GPU context per file; --pinned uses page-locked host buffers.\n");
     }
@@ -86,6 +93,8 @@ namespace {
                 o.checkMode = CheckMode::LowMemory;
             } else if (!strcmp(a, "--naive")) {
                 o.checkMode = CheckMode::Naive;
+            } else if (!strcmp(a, "--device")) {
+                o.deviceCheck = true;
             } else if (!strcmp(a, "--bwt")) {
                 o.bwt = true;
             } else if (!strcmp(a, "--both")) {
@@ -230,6 +239,41 @@ namespace {
         return kOk;
     }
 
+    // The fast check on the GPU: uploads the text and the array, returns what checkSuffixArray's
+    // fast mode returns (the first failing index and the reason).
+    CheckResult deviceCheckSuffixArray(const uint8_t* text, int32_t n, const int32_t* sa, bool& failedToRun) {
+        failedToRun = false;
+        if (n <= 0) {
+            return {true, -1, nullptr};
+        }
+        using Device = std::unique_ptr<void, void (*)(void*)>;
+        Device dText(sadcs_hip_device_alloc(static_cast<size_t>(n)), sadcs_hip_device_free);
+        Device dSA(sadcs_hip_device_alloc(static_cast<size_t>(n) * 4), sadcs_hip_device_free);
+        int64_t result[2] = {0, -1};
+        if (!dText || !dSA || sadcs_hip_memcpy(dText.get(), text, static_cast<size_t>(n), 1) != SADCS_OK ||
+            sadcs_hip_memcpy(dSA.get(), sa, static_cast<size_t>(n) * 4, 1) != SADCS_OK ||
+            sadcs_hip_check_device(dText.get(), n, dSA.get(), nullptr, 0, result) != SADCS_OK) {
+            printf("sadcs: device check could not run: %s\n", dText && dSA ? sadcs_hip_last_error() : "device allocation failed");
+            failedToRun = true;
+            return {false, -1, "device check could not run"};
+        }
+        switch (result[0]) {
+            case SADCS_CHECK_PASS:
+                return {true, -1, nullptr};
+            case SADCS_CHECK_RANGE:
+                return {false, result[1], "entry out of range"};
+            case SADCS_CHECK_DUPLICATE:
+                return {false, result[1], "duplicate entry"};
+            default:
+                return {false, result[1], "SA[index] and SA[index + 1] out of order"};
+        }
+    }
+
+    CheckResult checkArray(const Options& o, const uint8_t* text, int32_t n, const int32_t* sa, bool& failedToRun) {
+        failedToRun = false;
+        return o.deviceCheck ? deviceCheckSuffixArray(text, n, sa, failedToRun) : checkSuffixArray(text, n, sa, o.checkMode);
+    }
+
     int cmdCheck(const Options& o) {
         if (o.pathCount != 2) {
             usage();
@@ -270,12 +314,17 @@ namespace {
             printf("sadcs: '%s' has %zu entries, but '%s' has %d bytes\n", o.paths[1], sa.size(), o.paths[0], n);
             return kError;
         }
-        const CheckResult r = checkSuffixArray(text.data(), n, sa.data(), o.checkMode);
+        bool failedToRun;
+        const CheckResult r = checkArray(o, text.data(), n, sa.data(), failedToRun);
+        if (failedToRun) {
+            return kError;
+        }
         if (!r.ok) {
             printFailure(text.data(), n, sa.data(), r);
             return kError;
         }
-        printf("%s: suffix array OK (%d entries, %.3fs)\n", o.paths[1], n, omp_get_wtime() - start);
+        printf("%s: suffix array OK (%d entries%s, %.3fs)\n", o.paths[1], n, o.deviceCheck ? ", checked on the GPU" : "",
+               omp_get_wtime() - start);
         return kOk;
     }
 
@@ -313,9 +362,10 @@ namespace {
             }
             run.gpuSeconds = omp_get_wtime() - start;
             const double checkStart = omp_get_wtime();
-            const CheckResult r = checkSuffixArray(text.data(), n, sa.data(), o.checkMode);
+            bool failedToRun;
+            const CheckResult r = checkArray(o, text.data(), n, sa.data(), failedToRun);
             run.checkSeconds = omp_get_wtime() - checkStart;
-            if (!r.ok) {
+            if (!r.ok && !failedToRun) {
                 printFailure(text.data(), n, sa.data(), r);
             }
             run.ok = r.ok;

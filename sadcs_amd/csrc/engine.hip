@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/sadcs_hip.h"
+#include "check.h"
 #include "classes.h"
 #include "cover.h"
 #include "dev_common.h"
@@ -1318,7 +1319,8 @@ int queryDevice(sadcs_device& device) {
         setError("hipMemGetInfo: %s", hipGetErrorString(e));
         return SADCS_UNAVAILABLE;
     }
-    snprintf(device.name, sizeof device.name, "%s (%s)", properties.name, properties.gcnArchName);
+    // The marketing name can be empty where the runtime lacks the device-id table.
+    snprintf(device.name, sizeof device.name, "%s (%s)", properties.name[0] ? properties.name : "AMD GPU", properties.gcnArchName);
     device.compute_major = properties.major;
     device.compute_minor = properties.minor;
     device.total_bytes = static_cast<int64_t>(totalBytes);
@@ -2141,6 +2143,60 @@ int sadcs_hip_batch_build_device(sadcs_ctx* context, const void* d_texts, const 
         std::copy(first.begin(), first.end(), primaries);
     }
     return status;
+}
+
+int sadcs_hip_check_device(const void* d_text, int32_t n, const void* d_sa, const void* d_bwt, int32_t primary, int64_t* result) {
+    tLastError[0] = '\0';
+    if (!result) {
+        setError("invalid arguments");
+        return SADCS_FAILED;
+    }
+    result[0] = SADCS_CHECK_PASS;
+    result[1] = -1;
+    if (n <= 0) {
+        return SADCS_OK;
+    }
+    if (!d_text || !d_sa || (reinterpret_cast<uintptr_t>(d_sa) & 3) != 0) {
+        setError("the device text and a 4-byte aligned device suffix array are needed");
+        return SADCS_FAILED;
+    }
+    // inverse: n int32, then the first failing index of each check.
+    const size_t inverseBytes = (static_cast<size_t>(n) * sizeof(int32_t) + 255) / 256 * 256;
+    void* memory = nullptr;
+    SADCS_HIP(hipMalloc(&memory, inverseBytes + 256));
+    std::unique_ptr<void, void (*)(void*)> owner(memory, [](void* m) { (void)hipFree(m); });
+    auto* inverse = static_cast<int32_t*>(memory);
+    auto* first = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(memory) + inverseBytes);
+    const auto* text = static_cast<const uint8_t*>(d_text);
+    const auto* sa = static_cast<const int32_t*>(d_sa);
+    const uint32_t blocks = std::min<uint32_t>(kCheckBlocks, (static_cast<uint32_t>(n) + kCheckThreads - 1) / kCheckThreads);
+    SADCS_HIP(hipMemset(memory, 0xff, inverseBytes + kCheckCount * sizeof(uint32_t)));
+    checkRangeKernel<<<blocks, kCheckThreads>>>(sa, static_cast<uint32_t>(n), inverse, first);
+    checkOrderKernel<<<blocks, kCheckThreads>>>(text, sa, static_cast<uint32_t>(n), inverse, first);
+    if (d_bwt) {
+        checkBwtKernel<<<blocks, kCheckThreads>>>(text, sa, static_cast<uint32_t>(n), inverse, static_cast<const uint8_t*>(d_bwt), first);
+    }
+    SADCS_HIP(hipGetLastError());
+    uint32_t found[kCheckCount];
+    SADCS_HIP(hipMemcpy(found, first, sizeof found, hipMemcpyDeviceToHost));
+    // The first failing check in the reference's order: range, duplicates, order; then the BWT.
+    const int32_t categories[] = {SADCS_CHECK_RANGE, SADCS_CHECK_DUPLICATE, SADCS_CHECK_ORDER};
+    for (int c = 0; c < 3; c++) {
+        if (found[c] != UINT32_MAX) {
+            result[0] = categories[c];
+            result[1] = found[c];
+            return SADCS_OK;
+        }
+    }
+    if (d_bwt) {
+        if (static_cast<int64_t>(primary) != static_cast<int64_t>(found[kCheckRow0]) + 1) {
+            result[0] = SADCS_CHECK_PRIMARY;
+        } else if (found[kCheckBwt] != UINT32_MAX) {
+            result[0] = SADCS_CHECK_BWT;
+            result[1] = found[kCheckBwt];
+        }
+    }
+    return SADCS_OK;
 }
 
 int64_t sadcs_hip_batch_required_bytes(const int64_t* lengths, int32_t count) {

@@ -18,7 +18,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _PATH = os.path.join(_HERE, "libsadcs_work.so")
 if not os.path.exists(_PATH):
-    raise ImportError(f"{_PATH} is missing: build it with `python -m sadcs_amd.build`")
+    raise ImportError(f"{_PATH} is missing: build it with `python sadcs_amd/build.py`")
 _lib = ctypes.CDLL(_PATH)
 _u64, _i64, _P = ctypes.c_uint64, ctypes.c_int64, ctypes.c_void_p
 for _name in ("sadcs_gen_dna_uniform", "sadcs_gen_markov", "sadcs_gen_protein", "sadcs_gen_periodic"):
