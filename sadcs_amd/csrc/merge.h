@@ -621,56 +621,81 @@ inline uint32_t defaultTileSplitters(uint32_t g) {
     return records > spill + g ? static_cast<uint32_t>((records - spill) / g) : 1;
 }
 
-// Record x < record y, both in LDS. Cover 32 keeps all the characters a comparison can need in
-// the record and finds the tie offset and rank slots by bit arithmetic, so a comparison is at most
-// two dependent LDS round trips: the positions with the first words, then - only if those tie -
-// the other words and the two ranks together.
+// The fields of an LDS record that decide most comparisons (cover 32): a merge keeps them in
+// registers for its two heads, so a merge step loads one record's key instead of two.
+template <bool kBatch>
+struct HeadKey {
+    uint32_t position;
+    uint32_t tag;  // batch only
+    uint64_t word;
+};
+
+template <int V, bool kBatch>
+__device__ __forceinline__ HeadKey<kBatch> headKey(const LdsRecords<V, kBatch>& R, uint32_t x) {
+    HeadKey<kBatch> k;
+    k.position = R.position[x];
+    k.word = R.words[x];
+    k.tag = kBatch ? R.tag[x] : 0;
+    return k;
+}
+
+// Record x < record y (cover 32) given their keys; the other words and the ranks come from LDS
+// only if the first words tie. Cover 32 keeps all the characters a comparison can need in the
+// record and finds the tie offset and rank slots by bit arithmetic.
+template <int V, class Texts>
+__device__ __forceinline__ bool lessKeyed(const MergeCtx& c, const LdsRecords<V, Texts::kBatch>& R, uint32_t x, const HeadKey<Texts::kBatch>& kx, uint32_t y, const HeadKey<Texts::kBatch>& ky) {
+    static_assert(V == 32, "keys hold what a cover-32 comparison reads first");
+    const uint32_t cap = R.capacity;
+    const uint32_t i = kx.position, j = ky.position;
+    const uint64_t a0 = kx.word, b0 = ky.word;
+    uint32_t ra, rb, sameText = 1, textLess = 0;
+    if constexpr (Texts::kBatch) {
+        const uint32_t ta = kx.tag, tb = ky.tag;
+        sameText = (ta >> 9) == (tb >> 9);
+        textLess = (ta >> 9) < (tb >> 9);
+        ra = ta & 0x1FF;
+        rb = tb & 0x1FF;
+    } else {
+        ra = min(c.n - i, static_cast<uint32_t>(V));
+        rb = min(c.n - j, static_cast<uint32_t>(V));
+    }
+    const uint32_t wi = windowOf32(coverMask<V>(), i), wj = windowOf32(coverMask<V>(), j);
+    const uint32_t k = __builtin_ctz(wi & wj);
+    const uint32_t common = min(k, min(ra, rb));
+    const uint64_t m0 = common == 0 ? 0 : prefixMask(common);
+    bool decided = (a0 & m0) != (b0 & m0);
+    bool less = (a0 & m0) < (b0 & m0);
+    if (!decided) {
+        const bool rankTie = common == k && ra != k && rb != k;
+        uint32_t rankA = 0, rankB = 0;
+        if (rankTie) {
+            const uint32_t before = (1u << k) - 1;
+            rankA = R.ranks[__builtin_popcount(wi & before) * cap + x];
+            rankB = R.ranks[__builtin_popcount(wj & before) * cap + y];
+        }
+        if (common > 8) {
+#pragma unroll
+            for (int w = 1; w < kRecordWords; w++) {
+                const uint32_t rest = common > 8u * w ? common - 8u * w : 0u;
+                const uint64_t mask = rest == 0 ? 0 : prefixMask(rest);
+                const uint64_t av = R.words[w * cap + x] & mask, bv = R.words[w * cap + y] & mask;
+                less = decided ? less : av < bv;
+                decided = decided || av != bv;
+            }
+        }
+        if (!decided) {
+            less = rankTie ? rankA < rankB : i > j;  // i > j: one is a prefix of the other, the shorter is smaller
+        }
+    }
+    return sameText ? less : textLess;
+}
+
+// Record x < record y, both in LDS: at most two dependent LDS round trips for cover 32 (the keys,
+// then - only if those tie - the other words and the two ranks together).
 template <int V, class Texts>
 __device__ __forceinline__ bool lessLds(const MergeCtx& c, const TieTables& t, const LdsRecords<V, Texts::kBatch>& R, uint32_t x, uint32_t y) {
     if constexpr (V == 32) {
-        const uint32_t cap = R.capacity;
-        const uint32_t i = R.position[x], j = R.position[y];
-        const uint64_t a0 = R.words[x], b0 = R.words[y];
-        uint32_t ra, rb, sameText = 1, textLess = 0;
-        if constexpr (Texts::kBatch) {
-            const uint32_t ta = R.tag[x], tb = R.tag[y];
-            sameText = (ta >> 9) == (tb >> 9);
-            textLess = (ta >> 9) < (tb >> 9);
-            ra = ta & 0x1FF;
-            rb = tb & 0x1FF;
-        } else {
-            ra = min(c.n - i, static_cast<uint32_t>(V));
-            rb = min(c.n - j, static_cast<uint32_t>(V));
-        }
-        const uint32_t wi = windowOf32(coverMask<V>(), i), wj = windowOf32(coverMask<V>(), j);
-        const uint32_t k = __builtin_ctz(wi & wj);
-        const uint32_t common = min(k, min(ra, rb));
-        const uint64_t m0 = common == 0 ? 0 : prefixMask(common);
-        bool decided = (a0 & m0) != (b0 & m0);
-        bool less = (a0 & m0) < (b0 & m0);
-        if (!decided) {
-            const bool rankTie = common == k && ra != k && rb != k;
-            uint32_t rankA = 0, rankB = 0;
-            if (rankTie) {
-                const uint32_t before = (1u << k) - 1;
-                rankA = R.ranks[__builtin_popcount(wi & before) * cap + x];
-                rankB = R.ranks[__builtin_popcount(wj & before) * cap + y];
-            }
-            if (common > 8) {
-#pragma unroll
-                for (int w = 1; w < kRecordWords; w++) {
-                    const uint32_t rest = common > 8u * w ? common - 8u * w : 0u;
-                    const uint64_t mask = rest == 0 ? 0 : prefixMask(rest);
-                    const uint64_t av = R.words[w * cap + x] & mask, bv = R.words[w * cap + y] & mask;
-                    less = decided ? less : av < bv;
-                    decided = decided || av != bv;
-                }
-            }
-            if (!decided) {
-                less = rankTie ? rankA < rankB : i > j;  // i > j: one is a prefix of the other, the shorter is smaller
-            }
-        }
-        return sameText ? less : textLess;
+        return lessKeyed<V, Texts>(c, R, x, headKey<V, Texts::kBatch>(R, x), y, headKey<V, Texts::kBatch>(R, y));
     } else {
         uint32_t ra, rb;
         if constexpr (Texts::kBatch) {
@@ -745,24 +770,59 @@ __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const TieTables& t
             return;  // inconsistent bounds; the output check reports it
         }
         uint16_t headA = src[min(aStart + ia, last)], headB = src[min(aEnd + ib, last)];
-        while (o < stop) {
-            const uint16_t nextA = src[min(aStart + ia + 1, last)];
-            const uint16_t nextB = src[min(aEnd + ib + 1, last)];
-            bool takeA;
-            if (ib >= lengthB) {
-                takeA = true;
-            } else if (ia >= lengthA) {
-                takeA = false;
-            } else {
-                takeA = lessLds<V, Texts>(c, t, recs, headA, headB);
+        if constexpr (V == 32) {
+            // Heads' keys in registers; each step loads the key of the new head and the index
+            // after it (one LDS round trip on the dependent path). A head past the end of its run
+            // can be an unwritten slot: its key is loaded clamped and never used.
+            using Key = HeadKey<Texts::kBatch>;
+            Key keyA = headKey<V, Texts::kBatch>(recs, min<uint32_t>(headA, last));
+            Key keyB = headKey<V, Texts::kBatch>(recs, min<uint32_t>(headB, last));
+            uint16_t nextA = src[min(aStart + ia + 1, last)], nextB = src[min(aEnd + ib + 1, last)];
+            while (o < stop) {
+                bool takeA;
+                if (ib >= lengthB) {
+                    takeA = true;
+                } else if (ia >= lengthA) {
+                    takeA = false;
+                } else {
+                    takeA = lessKeyed<V, Texts>(c, recs, headA, keyA, headB, keyB);
+                }
+                dst[o++] = takeA ? headA : headB;
+                const uint16_t head = takeA ? nextA : nextB;
+                const Key key = headKey<V, Texts::kBatch>(recs, min<uint32_t>(head, last));
+                const uint16_t next = src[min(takeA ? aStart + ia + 2 : aEnd + ib + 2, last)];
+                if (takeA) {
+                    ia++;
+                    headA = head;
+                    keyA = key;
+                    nextA = next;
+                } else {
+                    ib++;
+                    headB = head;
+                    keyB = key;
+                    nextB = next;
+                }
             }
-            dst[o++] = takeA ? headA : headB;
-            if (takeA) {
-                ia++;
-                headA = nextA;
-            } else {
-                ib++;
-                headB = nextB;
+        } else {
+            while (o < stop) {
+                const uint16_t nextA = src[min(aStart + ia + 1, last)];
+                const uint16_t nextB = src[min(aEnd + ib + 1, last)];
+                bool takeA;
+                if (ib >= lengthB) {
+                    takeA = true;
+                } else if (ia >= lengthA) {
+                    takeA = false;
+                } else {
+                    takeA = lessLds<V, Texts>(c, t, recs, headA, headB);
+                }
+                dst[o++] = takeA ? headA : headB;
+                if (takeA) {
+                    ia++;
+                    headA = nextA;
+                } else {
+                    ib++;
+                    headB = nextB;
+                }
             }
         }
     }
@@ -842,6 +902,96 @@ __device__ __forceinline__ void writeTile(const MergeCtx& c, const ListSet& list
     }
 }
 
+// The records of a cover-32 tile from the lines, two lanes per record: random gathers cost per
+// request, and lanes of one instruction that hit the same line share one. The text lane loads
+// the three quads holding T[p - 1 .. p + 31] (bytes q .. q + 32 of the line, q = p mod 32) and
+// writes the four text words and the character before; the rank lane loads the quads holding
+// the record's seven ranks (dwords 16 + below(q) .. 22 + below(q) of the line, in three quads
+// when below(q) mod 4 = 3) and writes them, the position
+// and the index. Each lane decodes only what it loaded, in registers. Positions are the tile's
+// rows of the SA buffer.
+constexpr int kPairSlots = 5;  // records per lane pair in flight: a default tile (<= 1280) in one batch
+
+template <class Texts, int kFlags>
+__device__ __forceinline__ void loadRecordsFromLines(const MergeCtx& c, const Texts& texts, const LdsRecords<32, Texts::kBatch>& recs, const uint32_t* __restrict__ rowsIn, uint32_t size, uint8_t* before, uint16_t* indexA, uint32_t* errors) {
+    constexpr uint32_t kPairs = kMergeThreads / 2;
+    const bool textLane = (threadIdx.x & 1) == 0;
+    const uint32_t cap = recs.capacity;
+    for (uint32_t e0 = threadIdx.x / 2; e0 < size; e0 += kPairs * kPairSlots) {
+        uint32_t pos[kPairSlots];
+#pragma unroll
+        for (int h = 0; h < kPairSlots; h++) {
+            pos[h] = rowsIn[min(e0 + h * kPairs, size - 1)];
+        }
+        uint4 quad[kPairSlots][3];
+#pragma unroll
+        for (int h = 0; h < kPairSlots; h++) {
+            uint32_t p = pos[h];
+            if (p >= c.n) {
+                if (e0 + h * kPairs < size && textLane) {
+                    atomicOr(errors, 2u);
+                }
+                p = 0;
+                pos[h] = 0;
+            }
+            const uint32_t q = p & (kLinePositions - 1);
+            const uint4* line = c.lines + (static_cast<uint64_t>(p / kLinePositions) << 3);
+            const uint4* from = textLane ? line + (q >> 4) : line + 4 + (coverBelow<32>(q) >> 2);
+            quad[h][0] = from[0];
+            quad[h][1] = from[1];
+            quad[h][2] = from[2];
+        }
+#pragma unroll
+        for (int h = 0; h < kPairSlots; h++) {
+            const uint32_t e = e0 + h * kPairs;
+            if (e >= size) {
+                continue;
+            }
+            const uint32_t p = pos[h];
+            const uint32_t q = p & (kLinePositions - 1);
+            if (textLane) {
+                // T[p] is byte (q mod 16) + 1 of the 48 loaded bytes.
+                const uint4 q0 = quad[h][0], q1 = quad[h][1], q2 = quad[h][2];
+                const uint64_t w[6] = {q0.x | (uint64_t(q0.y) << 32), q0.z | (uint64_t(q0.w) << 32),
+                                       q1.x | (uint64_t(q1.y) << 32), q1.z | (uint64_t(q1.w) << 32),
+                                       q2.x | (uint64_t(q2.y) << 32), q2.z | (uint64_t(q2.w) << 32)};
+                const uint32_t offset = (q & 15) + 1;
+                const uint32_t wordShift = offset >> 3, shift = (offset & 7) * 8;
+                uint64_t v[5];
+#pragma unroll
+                for (int k = 0; k < 5; k++) {
+                    const uint64_t a1 = k + 1 < 6 ? w[k + 1] : 0;
+                    const uint64_t a2 = k + 2 < 6 ? w[k + 2] : 0;
+                    v[k] = wordShift == 0 ? w[k] : wordShift == 1 ? a1 : a2;
+                }
+#pragma unroll
+                for (int k = 0; k < kRecordWords; k++) {
+                    recs.words[k * cap + e] = __builtin_bswap64(shift ? (v[k] >> shift) | (v[k + 1] << (64 - shift)) : v[k]);
+                }
+                if constexpr ((kFlags & kOutBWT) != 0) {
+                    const uint32_t b = offset - 1;
+                    before[e] = p > 0 ? static_cast<uint8_t>(w[(b >> 3) & 1] >> ((b & 7) * 8)) : 0;
+                }
+            } else {
+                const uint32_t o = coverBelow<32>(q) & 3;
+                const uint4 r0 = quad[h][0], r1 = quad[h][1], r2 = quad[h][2];
+                const uint32_t x[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
+#pragma unroll
+                for (int k = 0; k < CoverSet<32>::kSize; k++) {
+                    recs.ranks[k * cap + e] = o == 0 ? x[k] : o == 1 ? x[k + 1] : o == 2 ? x[k + 2] : x[k + 3];
+                }
+                recs.position[e] = p;
+                if constexpr (Texts::kBatch) {
+                    const uint32_t segment = texts.segmentOf(p);
+                    const uint32_t remaining = c.segments[segment].end - p;
+                    recs.tag[e] = (segment << 9) | min(remaining, 32u);
+                }
+                indexA[e] = static_cast<uint16_t>(e);
+            }
+        }
+    }
+}
+
 // One tile per workgroup: load its records into LDS, merge, write.
 template <int V, class Texts, int kFlags>
 __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Texts texts, ListSet lists, const uint32_t* __restrict__ rows, uint32_t capacity, TileOut out, uint32_t* __restrict__ errors) {
@@ -878,10 +1028,17 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
     }
     const uint32_t size = segStart[lc];
 
-    // Load the records, kLoadSlots per thread at a time: every slot's position first, then every
+    bool fromLines = false;
+    if constexpr (V == kLinePositions && (kFlags & kInPlace) != 0) {
+        fromLines = c.lines != nullptr;
+        if (fromLines) {
+            loadRecordsFromLines<Texts, kFlags>(c, texts, recs, out.sa + outBase, size, before, indexA, errors);
+        }
+    }
+    // Otherwise kLoadSlots records per thread at a time: every slot's position first, then every
     // slot's record gathers, so a tile of up to kLoadSlots * kMergeThreads records costs two dependent
     // round trips (straight-line code, clamped indices); the BWT character before each record too.
-    {
+    if (!fromLines) {
         for (uint32_t e0 = threadIdx.x; e0 < size; e0 += kLoadSlots * kMergeThreads) {
             uint32_t pos[kLoadSlots];
             int l = 0;
