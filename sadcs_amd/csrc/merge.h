@@ -737,11 +737,13 @@ __device__ __forceinline__ bool lessLds(const MergeCtx& c, const TieTables& t, c
 // Outputs [o, o1) of one round of pairwise merges: run 2p and run 2p + 1 of src (bounds[0..runs])
 // merge into dst at the same place. Each output range starts with a merge-path search for its
 // first output, then merges sequentially with the two head indices in registers and the indices
-// after them loaded together with each comparison. src holds `capacity` entries.
-template <int V, class Texts>
+// after them loaded together with each comparison. src holds `capacity` entries; kIdentity: src
+// is the identity (the first round, runs still in list order) and is not read.
+template <int V, class Texts, bool kIdentity>
 __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const TieTables& t, const LdsRecords<V, Texts::kBatch>& recs, const uint16_t* src, uint16_t* dst, const uint32_t* bounds, uint32_t runs, uint32_t o, uint32_t o1) {
     const uint32_t pairs = (runs + 1) / 2;
     const uint32_t last = recs.capacity - 1;
+    auto at = [&](uint32_t x) -> uint16_t { return kIdentity ? static_cast<uint16_t>(x) : src[x]; };
     uint32_t j = 0;
     while (o < o1) {
         while (j + 1 < pairs && bounds[2 * (j + 1)] <= o) {
@@ -758,7 +760,7 @@ __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const TieTables& t
         uint32_t hi = min(d, lengthA);
         while (lo < hi) {
             const uint32_t mid = (lo + hi) / 2;
-            if (lessLds<V, Texts>(c, t, recs, src[aStart + mid], src[aEnd + d - 1 - mid])) {
+            if (lessLds<V, Texts>(c, t, recs, at(aStart + mid), at(aEnd + d - 1 - mid))) {
                 lo = mid + 1;
             } else {
                 hi = mid;
@@ -769,7 +771,7 @@ __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const TieTables& t
         if (stop <= o) {
             return;  // inconsistent bounds; the output check reports it
         }
-        uint16_t headA = src[min(aStart + ia, last)], headB = src[min(aEnd + ib, last)];
+        uint16_t headA = at(min(aStart + ia, last)), headB = at(min(aEnd + ib, last));
         if constexpr (V == 32) {
             // Heads' keys in registers; each step loads the key of the new head and the index
             // after it (one LDS round trip on the dependent path). A head past the end of its run
@@ -777,7 +779,7 @@ __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const TieTables& t
             using Key = HeadKey<Texts::kBatch>;
             Key keyA = headKey<V, Texts::kBatch>(recs, min<uint32_t>(headA, last));
             Key keyB = headKey<V, Texts::kBatch>(recs, min<uint32_t>(headB, last));
-            uint16_t nextA = src[min(aStart + ia + 1, last)], nextB = src[min(aEnd + ib + 1, last)];
+            uint16_t nextA = at(min(aStart + ia + 1, last)), nextB = at(min(aEnd + ib + 1, last));
             while (o < stop) {
                 bool takeA;
                 if (ib >= lengthB) {
@@ -790,7 +792,7 @@ __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const TieTables& t
                 dst[o++] = takeA ? headA : headB;
                 const uint16_t head = takeA ? nextA : nextB;
                 const Key key = headKey<V, Texts::kBatch>(recs, min<uint32_t>(head, last));
-                const uint16_t next = src[min(takeA ? aStart + ia + 2 : aEnd + ib + 2, last)];
+                const uint16_t next = at(min(takeA ? aStart + ia + 2 : aEnd + ib + 2, last));
                 if (takeA) {
                     ia++;
                     headA = head;
@@ -805,8 +807,8 @@ __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const TieTables& t
             }
         } else {
             while (o < stop) {
-                const uint16_t nextA = src[min(aStart + ia + 1, last)];
-                const uint16_t nextB = src[min(aEnd + ib + 1, last)];
+                const uint16_t nextA = at(min(aStart + ia + 1, last));
+                const uint16_t nextB = at(min(aEnd + ib + 1, last));
                 bool takeA;
                 if (ib >= lengthB) {
                     takeA = true;
@@ -837,14 +839,22 @@ __device__ __forceinline__ TieTables setupTables() {
 
 // Merges the runs of indices in src (run starts bounds[0..runs), bounds[runs] = size) in LDS by
 // rounds of pairwise merges; returns the buffer holding the merged order. afterRound() runs after
-// every round, between barriers, on every thread.
+// every round, between barriers, on every thread. identityFirst: src[i] = i, so the first round
+// need not read it.
 template <int V, class Texts, int kThreads, class AfterRound>
-__device__ __forceinline__ uint16_t* mergeRuns(const MergeCtx& c, const TieTables& tables, const LdsRecords<V, Texts::kBatch>& recs, uint16_t* src, uint16_t* dst, uint32_t* bounds, uint32_t runs, uint32_t size, AfterRound afterRound) {
+__device__ __forceinline__ uint16_t* mergeRuns(const MergeCtx& c, const TieTables& tables, const LdsRecords<V, Texts::kBatch>& recs, uint16_t* src, uint16_t* dst, uint32_t* bounds, uint32_t runs, uint32_t size, bool identityFirst, AfterRound afterRound) {
     const uint32_t chunk = max(3u, (size + kThreads - 1) / kThreads);
+    bool first = true;
     while (runs > 1) {
         for (uint64_t o = static_cast<uint64_t>(threadIdx.x) * chunk; o < size; o += static_cast<uint64_t>(kThreads) * chunk) {
-            mergeChunk<V, Texts>(c, tables, recs, src, dst, bounds, runs, static_cast<uint32_t>(o), static_cast<uint32_t>(min(o + chunk, static_cast<uint64_t>(size))));
+            const uint32_t o0 = static_cast<uint32_t>(o), o1 = static_cast<uint32_t>(min(o + chunk, static_cast<uint64_t>(size)));
+            if (first && identityFirst) {
+                mergeChunk<V, Texts, true>(c, tables, recs, src, dst, bounds, runs, o0, o1);
+            } else {
+                mergeChunk<V, Texts, false>(c, tables, recs, src, dst, bounds, runs, o0, o1);
+            }
         }
+        first = false;
         __syncthreads();
         const uint32_t pairs = (runs + 1) / 2;
         if (threadIdx.x == 0) {
@@ -1083,7 +1093,7 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
     }
     __syncthreads();
 
-    const uint16_t* order = mergeRuns<V, Texts, kMergeThreads>(c, tables, recs, indexA, indexB, bounds, static_cast<uint32_t>(lc), size, [] {});
+    const uint16_t* order = mergeRuns<V, Texts, kMergeThreads>(c, tables, recs, indexA, indexB, bounds, static_cast<uint32_t>(lc), size, true, [] {});
     writeTile<V, Texts, kFlags, kMergeThreads>(c, lists, recs, order, before, segStart, rowStart, outBase, size, out, errors);
 }
 
