@@ -30,8 +30,16 @@ namespace sadcs {
 template <int V>
 __device__ __forceinline__ uint32_t classOf(uint32_t p);
 
-// T[q], T[q-1], T[q-2], T[q-3] from the top byte down; bytes before `start` read as 0.
+// T[q], T[q-1], T[q-2], T[q-3] from the top byte down; bytes before `start` read as 0. That is
+// T[q-3 .. q] as a little-endian word: one 8-byte load from the 4-byte boundary at or below q - 3
+// holds it (a random gather costs per request, so one load, not four byte loads). The text buffer
+// is 16-byte aligned and padded past its end.
 __device__ __forceinline__ uint32_t characterKey(const uint8_t* __restrict__ bytes, uint32_t q, uint32_t start) {
+    if (q >= start + 3) {
+        const uint32_t a = (q - 3) & ~3u;
+        const uint2 v = *reinterpret_cast<const uint2*>(__builtin_assume_aligned(bytes + a, 4));
+        return __builtin_amdgcn_alignbyte(v.y, v.x, q - 3 - a);  // bytes q - 3 - a .. q - a of v
+    }
     uint32_t key = 0;
 #pragma unroll
     for (uint32_t i = 0; i < 4; i++) {
