@@ -727,8 +727,23 @@ class Pipeline {
                 makePassKeysKernel<V, Texts, false><<<blocksFor(u), 256, 0, m_stream>>>(
                     rp, m_texts, spec, codes, first, values[listBuffer], reinterpret_cast<const uint32_t*>(keys[listBuffer]), u, keys[y], values[y]);
             }
-            sb.current = y;
-            radixSort<uint64_t>(m_stream, sb, u, 0, ordinalBits + spec.subkeyBits);
+            // After the first pass the unresolved groups are mostly pairs: sort within groups, and
+            // fall back to the radix sort for a pass with a group too large for that.
+            bool sorted = false;
+            if (!first && ordinalBits > 0 && spec.subkeyBits < 64) {
+                SADCS_HIP(hipMemsetAsync(counts + 2, 0, sizeof(uint32_t), m_stream));
+                groupRankSortKernel<<<blocksFor(u), 256, 0, m_stream>>>(keys[y], values[y], u, spec.subkeyBits, keys[1 - y], values[1 - y], counts + 2);
+                SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords + 2, counts + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
+                SADCS_HIP(hipStreamSynchronize(m_stream));
+                if (m_ctx.hostWords[2] == 0) {
+                    sb.current = 1 - y;
+                    sorted = true;
+                }
+            }
+            if (!sorted) {
+                sb.current = y;
+                radixSort<uint64_t>(m_stream, sb, u, 0, ordinalBits + spec.subkeyBits);
+            }
             const int c = sb.current;
 
             MarkState ms{};

@@ -135,6 +135,46 @@ __global__ __launch_bounds__(256) void makePassKeysKernel(RankParams rp, Texts t
     valuesOut[i] = t;
 }
 
+// The sort of a pass after the first, when every group is small. The list arrives grouped: keys
+// are (ordinal << subkeyBits) | subkey with ordinals non-decreasing along the list (the previous
+// pass compacted its unresolved groups in sorted order). The stable sort of such a list puts
+// element i at its group's start plus the members with a smaller key, plus those with an equal key
+// before i: the same order as the stable radix sort, with each element reading only its group.
+// Unresolved groups are mostly pairs (a repeat and its source), so this replaces the several
+// digit passes of a 50- to 64-bit radix sort. A group of more than kGroupRankMax elements sets
+// *large and the caller sorts the pass by radix instead (the output is then ignored).
+constexpr uint32_t kGroupRankMax = 32;
+
+__global__ void groupRankSortKernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ values, uint32_t u, int subkeyBits, uint64_t* __restrict__ keysOut, uint32_t* __restrict__ valuesOut, uint32_t* __restrict__ large) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= u) {
+        return;
+    }
+    const uint64_t key = keys[i];
+    const uint64_t ordinal = key >> subkeyBits;
+    uint32_t start = i;
+    for (uint32_t step = 0; step < kGroupRankMax && start > 0 && (keys[start - 1] >> subkeyBits) == ordinal; step++) {
+        start--;
+    }
+    uint32_t end = i + 1;
+    for (uint32_t step = 0; step < kGroupRankMax && end < u && (keys[end] >> subkeyBits) == ordinal; step++) {
+        end++;
+    }
+    const bool leftOpen = start > 0 && (keys[start - 1] >> subkeyBits) == ordinal;
+    const bool rightOpen = end < u && (keys[end] >> subkeyBits) == ordinal;
+    if (leftOpen || rightOpen || end - start > kGroupRankMax) {
+        atomicOr(large, 1u);
+        return;
+    }
+    uint32_t rank = start;
+    for (uint32_t j = start; j < end; j++) {
+        const uint64_t other = keys[j];
+        rank += (other < key || (other == key && j < i)) ? 1u : 0u;
+    }
+    keysOut[rank] = key;
+    valuesOut[rank] = values[i];
+}
+
 // Four fields: last head (max), group start (max), unresolved before (sum), unresolved groups before (sum).
 using MarkVec = Vec<4, 0x3u>;
 constexpr int kMarkItems = 8;
