@@ -157,8 +157,9 @@ def batched_measurement(count, steps, tune=True):
             "merge_params": tuned or "defaults"}
 
 
-def served_measurement(requests, factors=(0.5, 0.8, 1.1)):
-    """Poisson request streams through the serving loop, against one-at-a-time host builds."""
+def served_measurement(requests, factors=(0.5, 0.8, 1.1, None)):
+    """Poisson request streams through the serving loop, against one-at-a-time host builds. A factor
+    of None submits every request at once: the server's capacity with full batches."""
     import threading
 
     import numpy as np
@@ -184,8 +185,8 @@ def served_measurement(requests, factors=(0.5, 0.8, 1.1)):
     rates = []
     rng = np.random.default_rng(7)
     for factor in factors:
-        rate = factor * capacity
-        gaps = rng.exponential(1.0 / rate, len(texts))
+        rate = factor * capacity if factor else float("inf")
+        gaps = rng.exponential(1.0 / rate, len(texts)) if factor else np.zeros(len(texts))
         results = [None] * len(texts)
         with sadcs_amd.Server(workers=2, max_batch_bytes=64 << 20) as server:
             ids = [None] * len(texts)
@@ -210,8 +211,10 @@ def served_measurement(requests, factors=(0.5, 0.8, 1.1)):
             elapsed = time.perf_counter() - start
             counters = server.counters()
         latency = np.array([r[3] for r in results]) * 1e3
-        rates.append({"offered_x_sequential": factor, "offered_requests_per_s": round(rate, 1),
-                      "mb_per_s": round(total / elapsed / 1e6, 1), "p50_ms": round(float(np.percentile(latency, 50)), 2),
+        rates.append({"offered_x_sequential": factor if factor else "all at once",
+                      "offered_requests_per_s": round(rate, 1) if factor else None,
+                      "mb_per_s": round(total / elapsed / 1e6, 1), "x_sequential": round(seq_seconds / elapsed, 2),
+                      "p50_ms": round(float(np.percentile(latency, 50)), 2),
                       "p99_ms": round(float(np.percentile(latency, 99)), 2),
                       "mean_batch": round(float(np.mean([r[5] for r in results])), 2), "batches": counters["batches"]})
         print(f"# served at {factor}x: {rates[-1]}", file=sys.stderr, flush=True)

@@ -14,11 +14,14 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/sadcs_hip.h"
@@ -434,7 +437,7 @@ struct sadcs_ctx {
 
 namespace sadcs {
 
-constexpr size_t kStagingChunk = size_t(8) << 20;
+constexpr size_t kStagingChunk = size_t(32) << 20;
 constexpr int64_t kMemoryReserve = int64_t(256) << 20;
 constexpr size_t kTextPadding = 64;
 
@@ -1352,6 +1355,99 @@ bool isPinned(const void* host) {
     return attributes.type == hipMemoryTypeHost;
 }
 
+// ---- host copies ---------------------------------------------------------------------------------
+//
+// Between pageable caller memory and the pinned staging chunks the host copies every byte, and one
+// thread's memcpy (about 13 GB/s measured, first-touch page faults of fresh result arrays included)
+// is well below what PCIe 5 x16 moves: returning a suffix array to pageable memory was bound by it.
+// A process-wide pool of threads copies each chunk in slices. Calls from several contexts (server
+// workers) take turns: they share the host's memory bandwidth anyway.
+class HostCopier {
+public:
+    static HostCopier& instance() {
+        static HostCopier pool;
+        return pool;
+    }
+
+    void copy(void* dst, const void* src, size_t bytes) {
+        if (bytes < kParallelMin || m_threads.empty()) {
+            memcpy(dst, src, bytes);
+            return;
+        }
+        std::lock_guard<std::mutex> turn(m_turn);
+        const size_t parts = m_threads.size() + 1;
+        const size_t slice = ((bytes + parts - 1) / parts + 63) / 64 * 64;
+        {
+            std::lock_guard<std::mutex> lock(m_mutex);
+            m_dst = static_cast<uint8_t*>(dst);
+            m_src = static_cast<const uint8_t*>(src);
+            m_bytes = bytes;
+            m_slice = slice;
+            m_pending = static_cast<int>(m_threads.size());
+            m_generation++;
+        }
+        m_start.notify_all();
+        memcpy(dst, src, std::min(slice, bytes));
+        std::unique_lock<std::mutex> lock(m_mutex);
+        m_done.wait(lock, [&] { return m_pending == 0; });
+    }
+
+private:
+    static constexpr size_t kParallelMin = size_t(1) << 20;
+
+    HostCopier() {
+        const unsigned cores = std::thread::hardware_concurrency();
+        const unsigned workers = std::min(7u, cores > 1 ? cores - 1 : 0u);
+        for (unsigned w = 0; w < workers; w++) {
+            m_threads.emplace_back(&HostCopier::work, this, w + 1);
+        }
+    }
+
+    ~HostCopier() {
+        {
+            std::lock_guard<std::mutex> lock(m_mutex);
+            m_stop = true;
+        }
+        m_start.notify_all();
+        for (std::thread& t : m_threads) {
+            t.join();
+        }
+    }
+
+    void work(size_t part) {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lock(m_mutex);
+        for (;;) {
+            m_start.wait(lock, [&] { return m_stop || m_generation != seen; });
+            if (m_stop) {
+                return;
+            }
+            seen = m_generation;
+            const size_t begin = std::min(part * m_slice, m_bytes), end = std::min(begin + m_slice, m_bytes);
+            uint8_t* dst = m_dst;
+            const uint8_t* src = m_src;
+            lock.unlock();
+            if (end > begin) {
+                memcpy(dst + begin, src + begin, end - begin);
+            }
+            lock.lock();
+            if (--m_pending == 0) {
+                m_done.notify_one();
+            }
+        }
+    }
+
+    std::vector<std::thread> m_threads;
+    std::mutex m_turn, m_mutex;
+    std::condition_variable m_start, m_done;
+    uint64_t m_generation = 0;
+    int m_pending = 0;
+    bool m_stop = false;
+    uint8_t* m_dst = nullptr;
+    const uint8_t* m_src = nullptr;
+    size_t m_bytes = 0, m_slice = 0;
+};
+
 // Host to device through two pinned chunks, or directly when the host buffer is pinned.
 int upload(sadcs_ctx& ctx, void* device, const void* host, size_t bytes, bool& pinned) {
     pinned = true;
@@ -1367,7 +1463,7 @@ int upload(sadcs_ctx& ctx, void* device, const void* host, size_t bytes, bool& p
         const size_t length = std::min(kStagingChunk, bytes - offset);
         const int b = static_cast<int>(k & 1);
         SADCS_HIP(hipEventSynchronize(ctx.stagingDone[b]));
-        memcpy(ctx.staging[b], static_cast<const uint8_t*>(host) + offset, length);
+        HostCopier::instance().copy(ctx.staging[b], static_cast<const uint8_t*>(host) + offset, length);
         SADCS_HIP(hipMemcpyAsync(static_cast<uint8_t*>(device) + offset, ctx.staging[b], length, hipMemcpyHostToDevice, ctx.stream));
         SADCS_HIP(hipEventRecord(ctx.stagingDone[b], ctx.stream));
     }
@@ -1401,7 +1497,7 @@ int download(sadcs_ctx& ctx, void* host, const void* device, size_t bytes) {
         const int b = static_cast<int>(k & 1);
         const size_t offset = k * kStagingChunk;
         SADCS_HIP(hipEventSynchronize(ctx.stagingDone[b]));
-        memcpy(static_cast<uint8_t*>(host) + offset, ctx.staging[b], std::min(kStagingChunk, bytes - offset));
+        HostCopier::instance().copy(static_cast<uint8_t*>(host) + offset, ctx.staging[b], std::min(kStagingChunk, bytes - offset));
     }
     return SADCS_OK;
 }
@@ -1522,6 +1618,7 @@ void fillStats(sadcs_stats* stats, const PhaseTimes& t, size_t deviceBytes, int 
 struct BuildRequest {
     Mode mode = Mode::Both;
     const uint8_t* hostText = nullptr;       // concatenated without gaps, lengths from the layout
+    const uint8_t* const* hostTexts = nullptr;  // or one pointer per text (a batch)
     const int64_t* hostLengths = nullptr;
     const uint8_t* deviceText = nullptr;
     uint32_t* deviceSA = nullptr;            // device outputs (device calls) or null
@@ -1565,7 +1662,7 @@ int buildWith(sadcs_ctx& ctx, const Layout& layout, const BuildRequest& rq, Text
             const uint8_t* src = rq.hostText;
             for (size_t i = 0; i < layout.segments.size(); i++) {
                 const size_t n = layout.segments[i].end - layout.segments[i].start;
-                SADCS_TRY(upload(ctx, in.text + layout.segments[i].start, src, n, pinned));
+                SADCS_TRY(upload(ctx, in.text + layout.segments[i].start, rq.hostTexts ? rq.hostTexts[i] : src, n, pinned));
                 src += n;
             }
         }
@@ -2244,11 +2341,23 @@ int64_t sadcs_hip_batch_required_bytes(const int64_t* lengths, int32_t count) {
 
 namespace sadcs {
 
+// A host array left uninitialised (results are copied over it whole).
+template <class T>
+struct HostArray {
+    std::unique_ptr<T[]> data;
+    size_t size = 0;
+
+    void allocate(size_t n) {
+        data.reset(n ? new T[n] : nullptr);
+        size = n;
+    }
+};
+
 struct ServerRequest {
     int64_t id = 0;
     std::vector<uint8_t> text;
-    std::vector<int32_t> sa;
-    std::vector<uint8_t> bwt;
+    HostArray<int32_t> sa;
+    HostArray<uint8_t> bwt;
     int32_t primary = -1;
     int status = SADCS_OK;
     std::string error;
@@ -2330,6 +2439,10 @@ class Server {
             }
             w->ctx->arenaBase = nullptr;  // never free a pool chunk as an arena
             destroyContext(w->ctx);
+            if (w->pinnedOut) {
+                (void)hipHostFree(w->pinnedOut);
+                w->pinnedOut = nullptr;
+            }
         }
         m_workers.clear();
         if (m_poolBase) {
@@ -2369,11 +2482,11 @@ class Server {
             setError("%s", r->error.c_str());
             return r->status;
         }
-        if (sa && !r->sa.empty()) {
-            memcpy(sa, r->sa.data(), r->sa.size() * sizeof(int32_t));
+        if (sa && r->sa.size) {
+            HostCopier::instance().copy(sa, r->sa.data.get(), r->sa.size * sizeof(int32_t));
         }
-        if (bwt && !r->bwt.empty()) {
-            memcpy(bwt, r->bwt.data(), r->bwt.size());
+        if (bwt && r->bwt.size) {
+            HostCopier::instance().copy(bwt, r->bwt.data.get(), r->bwt.size);
         }
         if (primary) {
             *primary = r->primary;
@@ -2402,6 +2515,8 @@ class Server {
  private:
     struct Worker {
         sadcs_ctx* ctx = nullptr;
+        uint8_t* pinnedOut = nullptr;  // batch outputs, downloaded without staging
+        size_t pinnedBytes = 0;
         std::thread thread;
         std::vector<std::shared_ptr<ServerRequest>> batch;
         uint8_t* chunk = nullptr;
@@ -2485,8 +2600,14 @@ class Server {
                 if (r->text.size() <= 1) {
                     r->admitted = std::chrono::steady_clock::now();
                     if (r->text.size() == 1) {
-                        r->sa.assign(m_wantSA ? 1 : 0, 0);
-                        r->bwt.assign(m_wantBWT ? 1 : 0, r->text[0]);
+                        r->sa.allocate(m_wantSA ? 1 : 0);
+                        r->bwt.allocate(m_wantBWT ? 1 : 0);
+                        if (m_wantSA) {
+                            r->sa.data[0] = 0;
+                        }
+                        if (m_wantBWT) {
+                            r->bwt.data[0] = r->text[0];
+                        }
                         r->primary = 1;
                     } else {
                         r->primary = 0;
@@ -2587,26 +2708,46 @@ class Server {
 
     void runBatch(Worker& w, std::vector<std::shared_ptr<ServerRequest>>& batch) {
         std::vector<int64_t> lengths;
+        std::vector<const uint8_t*> texts;
         uint64_t total = 0;
         for (auto& r : batch) {
             lengths.push_back(static_cast<int64_t>(r->text.size()));
+            texts.push_back(r->text.data());
             total += r->text.size();
         }
-        std::vector<uint8_t> joined;
-        joined.reserve(total);
-        for (auto& r : batch) {
-            joined.insert(joined.end(), r->text.begin(), r->text.end());
-        }
         const Layout layout = layoutFor(m_params, lengths.data(), static_cast<int32_t>(lengths.size()), batch.size() > 1);
-        std::vector<int32_t> sa(m_wantSA ? total : 0);
-        std::vector<uint8_t> bwt(m_wantBWT ? total : 0);
+        // Outputs land in the worker's pinned buffer (one DMA each, no staging copies), grown to
+        // the largest batch; without it, in an uninitialised pageable buffer.
+        const size_t saBytes = m_wantSA ? total * sizeof(int32_t) : 0, bwtBytes = m_wantBWT ? total : 0;
+        if (w.pinnedBytes < saBytes + bwtBytes) {
+            if (w.pinnedOut) {
+                (void)hipHostFree(w.pinnedOut);
+            }
+            w.pinnedOut = nullptr;
+            w.pinnedBytes = 0;
+            if (hipHostMalloc(reinterpret_cast<void**>(&w.pinnedOut), saBytes + bwtBytes, hipHostMallocDefault) == hipSuccess) {
+                w.pinnedBytes = saBytes + bwtBytes;
+            } else {
+                (void)hipGetLastError();
+                w.pinnedOut = nullptr;
+            }
+        }
+        HostArray<uint8_t> pageable;
+        uint8_t* out = w.pinnedOut;
+        if (!out) {
+            pageable.allocate(saBytes + bwtBytes);
+            out = pageable.data.get();
+        }
+        int32_t* sa = reinterpret_cast<int32_t*>(out);
+        uint8_t* bwt = out + saBytes;
         std::vector<int32_t> primaries(batch.size(), -1);
         BuildRequest rq;
         rq.mode = mode();
-        rq.hostText = joined.data();
+        rq.hostText = texts[0];  // a batch of one is built as a single text
+        rq.hostTexts = texts.data();
         rq.hostLengths = lengths.data();
-        rq.hostSA = m_wantSA ? sa.data() : nullptr;
-        rq.hostBWT = m_wantBWT ? bwt.data() : nullptr;
+        rq.hostSA = m_wantSA ? sa : nullptr;
+        rq.hostBWT = m_wantBWT ? bwt : nullptr;
         rq.hostPrimaries = primaries.data();
         w.ctx->arenaBase = w.chunk;
         w.ctx->arenaBytes = w.bytes;
@@ -2624,10 +2765,12 @@ class Server {
             r->error = error;
             if (status == SADCS_OK) {
                 if (m_wantSA) {
-                    r->sa.assign(sa.begin() + static_cast<std::ptrdiff_t>(offset), sa.begin() + static_cast<std::ptrdiff_t>(offset + n));
+                    r->sa.allocate(n);
+                    HostCopier::instance().copy(r->sa.data.get(), sa + offset, n * sizeof(int32_t));
                 }
                 if (m_wantBWT) {
-                    r->bwt.assign(bwt.begin() + static_cast<std::ptrdiff_t>(offset), bwt.begin() + static_cast<std::ptrdiff_t>(offset + n));
+                    r->bwt.allocate(n);
+                    HostCopier::instance().copy(r->bwt.data.get(), bwt + offset, n);
                     r->primary = primaries[i];
                 }
             }
