@@ -1084,7 +1084,7 @@ class Pipeline {
                 out.sa = ownSA;
             }
             if (status == SADCS_OK && !m_dry) {
-                tileMajorCopyKernel<<<tiles, 256, 0, m_stream>>>(lists, rows, out.sa);
+                tileMajorCopyKernel<<<(tiles + 3) / 4, 256, 0, m_stream>>>(lists, rows, tiles, out.sa);
             }
             // Stream order keeps the copy's reads of the lists ahead of any write to their memory.
             for (int d = 0; d < kClasses; d++) {

@@ -502,43 +502,66 @@ __device__ __forceinline__ void tileLayout(const ListSet& lists, const uint32_t*
 // ---- tile-major copy ----------------------------------------------------------------------
 
 // Copies every tile's parts of the lists to its output range, list after list.
-__global__ void tileMajorCopyKernel(ListSet lists, const uint32_t* __restrict__ rows, uint32_t* __restrict__ out) {
-    __shared__ uint32_t segStart[kMaxLists + 1];
-    __shared__ uint32_t rowStart[kMaxLists];
-    __shared__ uint32_t outBase, valid;
+__global__ __launch_bounds__(256) void tileMajorCopyKernel(ListSet lists, const uint32_t* __restrict__ rows, uint32_t tiles, uint32_t* __restrict__ out) {
+    // One wave per tile, four tiles per workgroup: a tile is ~1200 elements, and a workgroup per
+    // tile left the copy bound by workgroup dispatch rather than by HBM.
+    constexpr uint32_t kTileWaves = 4;
+    constexpr int kCopyItems = 19;  // a default tile (at most 1216 elements) in one batch of loads
+    __shared__ uint32_t segStart[kTileWaves][kMaxLists + 1];
+    __shared__ uint32_t rowStart[kTileWaves][kMaxLists];
     __shared__ const uint32_t* listPtr[kMaxLists];  // indexed per lane: LDS, not kernel-argument loads
-    const int lc = lists.count;
-    const uint64_t tile = blockIdx.x;
-
-    if (threadIdx.x < static_cast<uint32_t>(lc)) {
+    const uint32_t lc = static_cast<uint32_t>(lists.count);
+    const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const uint64_t tile = static_cast<uint64_t>(blockIdx.x) * kTileWaves + wave;
+    if (threadIdx.x < lc) {
         listPtr[threadIdx.x] = lists.ptr[threadIdx.x];
     }
-    tileLayout(lists, rows, tile, ~0u, segStart, rowStart, &outBase, &valid);
     __syncthreads();
-    if (!valid) {
+    if (tile >= tiles) {
         return;
     }
-
-    // Up to kCopyItems elements per thread: all loads first, then the stores (which the compiler
-    // must assume could alias the lists, so interleaving would cost a round trip per element).
-    constexpr int kCopyItems = 8;
-    const uint32_t size = segStart[lc];
-    for (uint32_t e0 = threadIdx.x; e0 < size; e0 += kCopyItems * blockDim.x) {
+    // The tile's rows of every list (lane l: list l), as tilePublish checks them.
+    uint32_t a = 0, b = 0;
+    if (lane <= lc) {
+        a = rows[tile * (lc + 1) + lane];
+        b = rows[(tile + 1) * (lc + 1) + lane];
+    }
+    const bool isList = lane < lc;
+    const bool ok = !isList || (a <= b && b <= lists.length[lane]);
+    const uint32_t length = isList && ok ? b - a : 0;
+    const uint32_t inclusive = waveInclusive<OpSum>(length);
+    const uint32_t base = readLane(a, lc), end = readLane(b, lc), size = readLane(inclusive, lc);
+    if (!__all(ok) || base > end || end - base != size || end > lists.indexBase[lc]) {
+        return;  // inconsistent rows: the merge kernel reports them
+    }
+    if (lane <= lc) {
+        segStart[wave][lane] = inclusive - length;
+    }
+    if (isList) {
+        rowStart[wave][lane] = a;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t* seg = segStart[wave];
+    const uint32_t* row = rowStart[wave];
+    // All loads first, then the stores (which the compiler must assume could alias the lists).
+    for (uint32_t e0 = lane; e0 < size; e0 += kCopyItems * kWave) {
         uint32_t v[kCopyItems];
-        int l = 0;
+        uint32_t l = 0;
 #pragma unroll
         for (int j = 0; j < kCopyItems; j++) {
-            const uint32_t e = min(e0 + j * blockDim.x, size - 1);
-            while (l + 1 < lc && e >= segStart[l + 1]) {
+            const uint32_t e = min(e0 + j * kWave, size - 1);
+            while (l + 1 < lc && e >= seg[l + 1]) {
                 l++;
             }
-            v[j] = listPtr[l][rowStart[l] + (e - segStart[l])];
+            v[j] = listPtr[l][row[l] + (e - seg[l])];
         }
 #pragma unroll
         for (int j = 0; j < kCopyItems; j++) {
-            const uint32_t e = e0 + j * blockDim.x;
+            const uint32_t e = e0 + j * kWave;
             if (e < size) {
-                out[outBase + e] = v[j];
+                out[base + e] = v[j];
             }
         }
     }
