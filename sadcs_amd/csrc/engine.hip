@@ -102,23 +102,42 @@ __global__ void fillBytesKernel(uint8_t* __restrict__ out, uint64_t count, uint8
     }
 }
 
-// Byte histogram of the text positions that belong to a text.
+// The bytes that occur in the texts, as a 256-bit mask (bit b mod 32 of word b / 32): the dense
+// codes need only which bytes occur, not how often. Each thread reads 16-byte chunks (the text
+// buffer is 16-byte aligned and padded) and keeps the mask in registers; gap bytes between the
+// texts of a batch are masked off (a chunk never spans a text's start, which is 256-aligned).
 template <class Texts>
-__global__ void byteHistogramKernel(const uint8_t* __restrict__ bytes, uint32_t span, Texts texts, uint32_t* __restrict__ counts) {
-    __shared__ uint32_t local[256];
-    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) {
-        local[x] = 0;
-    }
-    __syncthreads();
-    for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < span; p += uint64_t(gridDim.x) * blockDim.x) {
-        if (!Texts::kBatch || p < texts.endOf(static_cast<uint32_t>(p))) {
-            atomicAdd(&local[bytes[p]], 1u);
+__global__ __launch_bounds__(256) void bytePresenceKernel(const uint4* __restrict__ chunks, uint32_t span, Texts texts, uint32_t* __restrict__ present) {
+    uint32_t mask[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t count = static_cast<uint32_t>((static_cast<uint64_t>(span) + 15) / 16);
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < count; c += gridDim.x * blockDim.x) {
+        const uint32_t p0 = c * 16;
+        const uint32_t end = min(texts.endOf(p0), span);
+        const uint32_t valid = p0 < end ? min(16u, end - p0) : 0u;
+        const uint4 q = chunks[c];
+        const uint32_t words[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++) {
+            const uint32_t b = (words[k >> 2] >> ((k & 3) * 8)) & 255u;
+            const uint32_t bit = k < valid ? 1u << (b & 31) : 0u;
+#pragma unroll
+            for (uint32_t x = 0; x < 8; x++) {
+                mask[x] |= (b >> 5) == x ? bit : 0u;
+            }
         }
     }
-    __syncthreads();
-    for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) {
-        if (local[x]) {
-            atomicAdd(&counts[x], local[x]);
+#pragma unroll
+    for (uint32_t x = 0; x < 8; x++) {
+        for (int offset = kWave / 2; offset > 0; offset /= 2) {
+            mask[x] |= __shfl_xor(mask[x], offset);
+        }
+    }
+    if (threadIdx.x % kWave == 0) {
+#pragma unroll
+        for (uint32_t x = 0; x < 8; x++) {
+            if (mask[x]) {
+                atomicOr(&present[x], mask[x]);
+            }
         }
     }
 }
@@ -647,17 +666,21 @@ class Pipeline {
         if (status == SADCS_OK) status = m_arena.allocate(sortAggregates, radixAggregateBytes(M) / sizeof(Sum1) + 1);
         if (status == SADCS_OK) status = m_arena.allocate(markAggregates, scanScratchBytes<MarkVec, kMarkItems>(M) / sizeof(MarkVec) + 1);
         if (status == SADCS_OK) status = m_arena.allocate(counts, 4);
-        if (status == SADCS_OK) status = m_arena.allocate(byteCounts, 256);
+        if (status == SADCS_OK) status = m_arena.allocate(byteCounts, 8);
         if (status != SADCS_OK || m_dry || M == 0) {
             cleanup();
             return status;
         }
 
         // The alphabet: dense codes 1..sigma for the bytes present.
-        fillWords(byteCounts, 256, 0);
-        byteHistogramKernel<Texts><<<std::min<unsigned>(blocksFor(m_job.span), 2048), 256, 0, m_stream>>>(m_job.text, m_job.span, m_texts, byteCounts);
-        SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords, byteCounts, 256 * sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
+        fillWords(byteCounts, 8, 0);
+        bytePresenceKernel<Texts><<<std::min<unsigned>(blocksFor((m_job.span + 15) / 16), 2048), 256, 0, m_stream>>>(
+            reinterpret_cast<const uint4*>(m_job.text), m_job.span, m_texts, byteCounts);
+        SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords + 256, byteCounts, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
         SADCS_HIP(hipStreamSynchronize(m_stream));
+        for (int b = 0; b < 256; b++) {
+            m_ctx.hostWords[b] = (m_ctx.hostWords[256 + b / 32] >> (b % 32)) & 1u;
+        }
         CodeTable codes{};
         int sigma = 0;
         for (int b = 0; b < 256; b++) {
