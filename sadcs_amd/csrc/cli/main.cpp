@@ -3,6 +3,11 @@
  * commands build|-b, check|-c, test|-t, bench, selftest with --bwt, --gpu, --cover, --threads,
  * --runs, --context, --pinned, --recursive, --lowmem, --naive, --verbose).
  *
+ * bench --batch builds every file as one batch (sadcs_hip_batch_build) and compares it with one
+ * file at a time; bench --serve submits every file at once to the continuous-batching server
+ * (sadcs_hip_server_*). build --pinned writes the output files straight from pinned host buffers.
+ * None of the three has a counterpart in the reference.
+ *
  * --device (check and test, suffix arrays) runs the fast check on the GPU instead of the host
  * (sadcs_hip_check_device: the same checks, the same first failing index); it has no counterpart
  * in the reference.
@@ -48,6 +53,8 @@ namespace {
         bool context = false;
         bool pinned = false;
         bool deviceCheck = false;
+        bool batch = false;
+        bool serve = false;
         int runs = 5;
     };
 
@@ -59,10 +66,12 @@ namespace {
             "  sadcs build|-b <in> <out.sa> [--cover 32|128] [--gpu]\n"
             "  sadcs build|-b --bwt <in> <out.bwt> [--cover 32|128] [--gpu]\n"
             "  sadcs build|-b --both <in> <out.sa>      (also writes <out.sa>.bwt, one merge)\n"
+            "  sadcs build|-b ... --pinned              (outputs in pinned host memory, written from there)\n"
             "  sadcs check|-c <in> <in.sa> [--lowmem | --naive | --device] [--threads N]\n"
             "  sadcs check|-c --bwt <in> <in.bwt>\n"
             "  sadcs test|-t <in|dir> [--bwt] [--cover 32|128] [--threads N] [--recursive] [--device]\n"
             "  sadcs bench <in|dir> [--bwt | --both] [--runs N] [--context] [--pinned] [--recursive]\n"
+            "  sadcs bench <dir> --batch | --serve [--bwt | --both] [--runs N] [--recursive]\n"
             "  sadcs selftest [--verbose]\n"
             "\n"
             "The .sa file is n little-endian int32 entries (divsufsort's mksary format).\n"
@@ -70,7 +79,9 @@ namespace {
             "test builds on the GPU and checks the result with the independent checker.\n"
             "--device runs the suffix-array check on the GPU (text and array uploaded; 4n bytes of device memory).\n"
             "bench times the build host to host: a warmup, then the best of --runs, with each phase.\n"
-            "--context reuses one GPU context per file; --pinned uses page-locked host buffers.\n");
+            "--context reuses one GPU context per file; --pinned uses page-locked host buffers.\n"
+            "--batch builds all files in one batched construction; --serve submits them all at once to the\n"
+            "serving loop; both compare with one file at a time and check that the outputs are equal.\n");
     }
 
     bool parse(int argc, char** argv, Options& o) {
@@ -95,6 +106,10 @@ namespace {
                 o.checkMode = CheckMode::Naive;
             } else if (!strcmp(a, "--device")) {
                 o.deviceCheck = true;
+            } else if (!strcmp(a, "--batch")) {
+                o.batch = true;
+            } else if (!strcmp(a, "--serve")) {
+                o.serve = true;
             } else if (!strcmp(a, "--bwt")) {
                 o.bwt = true;
             } else if (!strcmp(a, "--both")) {
@@ -208,14 +223,26 @@ namespace {
         const double start = omp_get_wtime();
         GpuStats stats;
         int32_t primary = 0;
-        std::vector<int32_t> sa(o.bwt && !o.both ? 0 : static_cast<size_t>(n));
-        std::vector<uint8_t> bwt;
-        if (o.both) {
-            bwt.resize(static_cast<size_t>(n));
+        const bool wantSA = !o.bwt || o.both;
+        // Outputs left uninitialised (every entry is written); with --pinned in page-locked memory,
+        // so the device writes them with one DMA each and the files are written from there.
+        using Host = std::unique_ptr<void, void (*)(void*)>;
+        auto hostBuffer = [&](size_t bytes) {
+            if (o.pinned) {
+                return Host(bytes ? gpuAllocatePinned(bytes) : nullptr, gpuFreePinned);
+            }
+            return Host(bytes ? ::operator new(bytes) : nullptr, [](void* q) { ::operator delete(q); });
+        };
+        Host saBuffer = hostBuffer(wantSA ? static_cast<size_t>(n) * sizeof(int32_t) : 0);
+        Host bwtBuffer = hostBuffer(o.both ? static_cast<size_t>(n) : 0);
+        if ((wantSA && n > 0 && !saBuffer) || (o.both && n > 0 && !bwtBuffer)) {
+            printf("sadcs: cannot allocate %shost memory for the outputs\n", o.pinned ? "pinned " : "");
+            return kError;
         }
+        int32_t* sa = static_cast<int32_t*>(saBuffer.get());
         // --bwt alone writes the BWT in place of the host text: n bytes in all (main.cpp:351-361).
-        uint8_t* bwtOut = o.both ? bwt.data() : o.bwt ? text.data() : nullptr;
-        if (!report(construct(o, nullptr, text.data(), n, sa.empty() ? nullptr : sa.data(), bwtOut, primary, stats))) {
+        uint8_t* bwtOut = o.both ? static_cast<uint8_t*>(bwtBuffer.get()) : o.bwt ? text.data() : nullptr;
+        if (!report(construct(o, nullptr, text.data(), n, wantSA ? sa : nullptr, bwtOut, primary, stats))) {
             return kError;
         }
         const double seconds = omp_get_wtime() - start;
@@ -223,10 +250,10 @@ namespace {
         if (o.bwt && !o.both) {
             written = writeBwtFile(o.paths[1], text.data(), n, primary);
         } else {
-            written = writeWholeFile(o.paths[1], sa.data(), sa.size() * sizeof(int32_t));
+            written = writeWholeFile(o.paths[1], sa, static_cast<size_t>(n) * sizeof(int32_t));
             if (o.both) {
                 const std::string bwtPath = std::string(o.paths[1]) + ".bwt";
-                written = written && writeBwtFile(bwtPath.c_str(), bwt.data(), n, primary);
+                written = written && writeBwtFile(bwtPath.c_str(), bwtOut, n, primary);
             }
         }
         if (!written) {
@@ -480,10 +507,180 @@ namespace {
         return true;
     }
 
+    struct Corpus {
+        std::vector<std::string> names;
+        std::vector<uint8_t> joined;      // the texts back to back
+        std::vector<int64_t> lengths;
+        std::vector<int64_t> offsets;
+        int64_t total = 0;
+    };
+
+    bool loadCorpus(const std::vector<std::filesystem::path>& files, Corpus& c) {
+        for (const auto& file : files) {
+            std::vector<uint8_t> text;
+            int32_t n;
+            if (!loadText(file.string().c_str(), text, n)) {
+                return false;
+            }
+            if (n < 1) {
+                continue;  // empty files have nothing to build
+            }
+            c.names.push_back(file.string());
+            c.offsets.push_back(c.total);
+            c.lengths.push_back(n);
+            c.joined.insert(c.joined.end(), text.begin(), text.begin() + n);
+            c.total += n;
+        }
+        return true;
+    }
+
+    // One file at a time through the reference API, best of the runs per file; outputs kept for
+    // the comparison.
+    bool oneAtATime(const Options& o, const Corpus& c, std::vector<int32_t>& sa, std::vector<uint8_t>& bwt, std::vector<int32_t>& primaries, double& seconds) {
+        const bool wantSA = !o.bwt || o.both, wantBWT = o.bwt || o.both;
+        sa.resize(wantSA ? c.total : 0);
+        bwt.resize(wantBWT ? c.total : 0);
+        primaries.assign(c.lengths.size(), -1);
+        seconds = 0;
+        for (size_t i = 0; i < c.lengths.size(); i++) {
+            const int32_t n = static_cast<int32_t>(c.lengths[i]);
+            double best = 1e30;
+            for (int run = 0; run <= o.runs; run++) {
+                GpuStats stats;
+                int32_t primary = 0;
+                const double start = omp_get_wtime();
+                if (!report(construct(o, nullptr, c.joined.data() + c.offsets[i], n, wantSA ? sa.data() + c.offsets[i] : nullptr,
+                                      wantBWT ? bwt.data() + c.offsets[i] : nullptr, primary, stats))) {
+                    return false;
+                }
+                if (run > 0) {
+                    best = std::min(best, omp_get_wtime() - start);
+                }
+                primaries[i] = primary;
+            }
+            seconds += best;
+        }
+        return true;
+    }
+
+    bool sameOutputs(const std::vector<int32_t>& sa, const std::vector<uint8_t>& bwt, const std::vector<int32_t>& primaries,
+                     const int32_t* sa2, const uint8_t* bwt2, const int32_t* primaries2) {
+        return (sa.empty() || memcmp(sa.data(), sa2, sa.size() * sizeof(int32_t)) == 0) &&
+               (bwt.empty() || (memcmp(bwt.data(), bwt2, bwt.size()) == 0 &&
+                                memcmp(primaries.data(), primaries2, primaries.size() * sizeof(int32_t)) == 0));
+    }
+
+    int benchBatch(const Options& o, const Corpus& c, const std::vector<int32_t>& seqSA, const std::vector<uint8_t>& seqBWT,
+                   const std::vector<int32_t>& seqPrimaries, double seqSeconds) {
+        const bool wantSA = !o.bwt || o.both, wantBWT = o.bwt || o.both;
+        std::vector<int32_t> sa(wantSA ? c.total : 0);
+        std::vector<uint8_t> bwt(wantBWT ? c.total : 0);
+        std::vector<int32_t> primaries(c.lengths.size(), -1);
+        double best = 1e30;
+        for (int run = 0; run <= o.runs; run++) {
+            sadcs_stats stats{};
+            const double start = omp_get_wtime();
+            if (sadcs_hip_batch_build(nullptr, c.joined.data(), c.lengths.data(), static_cast<int32_t>(c.lengths.size()),
+                                      wantSA ? sa.data() : nullptr, wantBWT ? bwt.data() : nullptr, primaries.data(), &stats) != SADCS_OK) {
+                printf("sadcs: batch build failed: %s\n", sadcs_hip_last_error());
+                return kError;
+            }
+            if (run > 0) {
+                best = std::min(best, omp_get_wtime() - start);
+            }
+        }
+        const bool equal = sameOutputs(seqSA, seqBWT, seqPrimaries, sa.data(), bwt.data(), primaries.data());
+        printf("batch of %zu files, %lld bytes: %.1f ms (%.1f MB/s); one at a time %.1f ms (%.1f MB/s): %.2fx; outputs %s\n",
+               c.lengths.size(), static_cast<long long>(c.total), best * 1e3, mibPerSecond(c.total, best), seqSeconds * 1e3,
+               mibPerSecond(c.total, seqSeconds), seqSeconds / best, equal ? "equal" : "DIFFER");
+        return equal ? kOk : kError;
+    }
+
+    int benchServe(const Options& o, const Corpus& c, const std::vector<int32_t>& seqSA, const std::vector<uint8_t>& seqBWT,
+                   const std::vector<int32_t>& seqPrimaries, double seqSeconds) {
+        const bool wantSA = !o.bwt || o.both, wantBWT = o.bwt || o.both;
+        sadcs_server_params params{};
+        params.cover = o.cover;
+        params.want_sa = wantSA;
+        params.want_bwt = wantBWT;
+        sadcs_server* server = nullptr;
+        if (sadcs_hip_server_create(&params, &server) != SADCS_OK) {
+            printf("sadcs: cannot start the server: %s\n", sadcs_hip_last_error());
+            return kError;
+        }
+        std::vector<int32_t> sa(wantSA ? c.total : 0);
+        std::vector<uint8_t> bwt(wantBWT ? c.total : 0);
+        std::vector<int32_t> primaries(c.lengths.size(), -1);
+        std::vector<double> latency(c.lengths.size());
+        std::vector<int64_t> ids;
+        const double start = omp_get_wtime();
+        for (size_t i = 0; i < c.lengths.size(); i++) {
+            ids.push_back(sadcs_hip_server_submit(server, c.joined.data() + c.offsets[i], c.lengths[i]));
+        }
+        bool ok = true;
+        for (size_t i = 0; i < ids.size(); i++) {
+            double queued = 0;
+            int32_t batchSize = 0;
+            ok = ok && ids[i] >= 0 &&
+                 sadcs_hip_server_wait(server, ids[i], wantSA ? sa.data() + c.offsets[i] : nullptr, wantBWT ? bwt.data() + c.offsets[i] : nullptr,
+                                       &primaries[i], &latency[i], &queued, &batchSize) == SADCS_OK;
+        }
+        const double seconds = omp_get_wtime() - start;
+        int64_t counters[5] = {};
+        sadcs_hip_server_counters(server, counters);
+        sadcs_hip_server_destroy(server);
+        if (!ok) {
+            printf("sadcs: a request failed: %s\n", sadcs_hip_last_error());
+            return kError;
+        }
+        std::sort(latency.begin(), latency.end());
+        auto percentile = [&](double q) { return latency[std::min(latency.size() - 1, static_cast<size_t>(q * latency.size()))] * 1e3; };
+        const bool equal = sameOutputs(seqSA, seqBWT, seqPrimaries, sa.data(), bwt.data(), primaries.data());
+        printf("served %zu files at once, %lld bytes: %.1f ms (%.1f MB/s) in %lld batches; latency p50 %.1f ms, p99 %.1f ms; "
+               "one at a time %.1f ms (%.1f MB/s): %.2fx; outputs %s\n",
+               c.lengths.size(), static_cast<long long>(c.total), seconds * 1e3, mibPerSecond(c.total, seconds), static_cast<long long>(counters[1]),
+               percentile(0.5), percentile(0.99), seqSeconds * 1e3, mibPerSecond(c.total, seqSeconds), seqSeconds / seconds, equal ? "equal" : "DIFFER");
+        return equal ? kOk : kError;
+    }
+
     int cmdBench(const Options& o) {
         if (o.pathCount != 1) {
             usage();
             return kError;
+        }
+        if (o.batch || o.serve) {
+            std::error_code error;
+            std::vector<std::filesystem::path> files;
+            if (std::filesystem::is_directory(o.paths[0], error)) {
+                if (!listFiles(o.paths[0], o.recursive, files)) {
+                    printf("sadcs: cannot list directory '%s'\n", o.paths[0]);
+                    return kError;
+                }
+            } else {
+                files.push_back(o.paths[0]);
+            }
+            Corpus corpus;
+            if (!loadCorpus(files, corpus)) {
+                return kError;
+            }
+            if (corpus.lengths.empty()) {
+                printf("sadcs: no non-empty files in '%s'\n", o.paths[0]);
+                return kError;
+            }
+            std::vector<int32_t> sa, primaries;
+            std::vector<uint8_t> bwt;
+            double seqSeconds = 0;
+            if (!oneAtATime(o, corpus, sa, bwt, primaries, seqSeconds)) {
+                return kError;
+            }
+            int status = kOk;
+            if (o.batch) {
+                status = benchBatch(o, corpus, sa, bwt, primaries, seqSeconds);
+            }
+            if (o.serve && status == kOk) {
+                status = benchServe(o, corpus, sa, bwt, primaries, seqSeconds);
+            }
+            return status;
         }
         std::error_code error;
         std::vector<std::filesystem::path> files;

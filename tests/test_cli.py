@@ -197,3 +197,33 @@ def test_test_command_over_a_directory(gpu, tmp_path):
 def test_selftest_passes(gpu):
     rc, out = run("selftest", timeout=600)
     assert rc == 0, out[-2000:]
+
+
+@pytest.mark.gpu
+def test_bench_batch_and_serve_equal_one_at_a_time(gpu, tmp_path):
+    from sadcs_amd import workloads
+    corpus = tmp_path / "corpus"
+    corpus.mkdir()
+    rng = np.random.default_rng(3)
+    for i, n in enumerate(rng.integers(1, 200_000, 12)):
+        data = workloads.markov(int(n), 40 + i) if i % 2 else workloads.dna_like(int(n), 60 + i)
+        (corpus / f"t{i:02d}.txt").write_bytes(data.tobytes())
+    (corpus / "empty.txt").write_bytes(b"")
+    (corpus / "one.txt").write_bytes(b"z")
+    rc, out = run("bench", corpus, "--batch", "--serve", "--both", "--runs", "1", timeout=300)
+    assert rc == 0, out
+    assert "batch of 13 files" in out and "served 13 files" in out, out
+    assert out.count("outputs equal") == 2, out
+
+
+@pytest.mark.gpu
+def test_build_from_pinned_buffers(gpu, tmp_path):
+    from sadcs_amd import workloads
+    text = workloads.markov(300_000, 17).tobytes()
+    path = tmp_path / "in.txt"
+    path.write_bytes(text)
+    want = oracle.suffix_array(text)
+    rc, out = run("build", "--both", path, tmp_path / "out.sa", "--pinned")
+    assert rc == 0, out
+    assert np.array_equal(np.fromfile(tmp_path / "out.sa", dtype="<i4"), want)
+    assert read_bwt(str(tmp_path / "out.sa") + ".bwt") == oracle.bwt_from_sa(text, want)
