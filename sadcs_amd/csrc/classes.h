@@ -65,6 +65,12 @@ struct ClassWalkSource {
     __device__ __forceinline__ bool operator()(uint32_t k, K& key, uint32_t& value) const {
         uint32_t w, start, segment;
         bool fromEnd = k < segments;
+        // Both list loads first and unconditionally (index clamped for the ends; the lists hold at
+        // least one entry): the sort kernels issue every item's loads before using any, which a
+        // load behind the class test would serialise into a second round trip per item.
+        const uint32_t index = fromEnd ? 0 : k - segments;
+        const uint32_t listed = previous[index];
+        const uint32_t listedChars = previousKeys ? static_cast<uint32_t>(previousKeys[index]) : 0u;
         if (fromEnd) {
             if constexpr (Texts::kBatch) {
                 const Segment s = segmentTable[k];
@@ -76,7 +82,7 @@ struct ClassWalkSource {
             }
             segment = k;
         } else {
-            w = previous[k - segments];
+            w = listed;
             if (w >= n) {
                 return false;  // not a position: drop it (the class then comes out short and fails its check)
             }
@@ -92,9 +98,7 @@ struct ClassWalkSource {
             return false;
         }
         const uint32_t q = w - 1;
-        const uint32_t chars = (previousKeys && !fromEnd)
-            ? static_cast<uint32_t>(previousKeys[k - segments]) << 8
-            : characterKey(bytes, q, start);
+        const uint32_t chars = (previousKeys && !fromEnd) ? listedChars << 8 : characterKey(bytes, q, start);
         if constexpr (sizeof(K) == 8) {
             key = (static_cast<K>(segment) << 32) | chars;
         } else {
