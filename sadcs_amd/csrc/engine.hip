@@ -850,8 +850,8 @@ class Pipeline {
         auto cleanup = [&]() {
             unscratch(walkValues);
             unscratch(walkKeys);
-            m_arena.release(aggregates);
-            m_arena.release(hist);
+            unscratch(aggregates);
+            unscratch(hist);
             unscratch(tmpValues);
             unscratch(tmpKeys);
             unscratch(keys[1]);
@@ -861,8 +861,10 @@ class Pipeline {
         if (status == SADCS_OK) status = scratch(keys[1], std::max<uint64_t>(largest, 1));
         if (kBatch && status == SADCS_OK) status = scratch(tmpKeys, std::max<uint64_t>(largest, 1));
         if (kBatch && status == SADCS_OK) status = scratch(tmpValues, std::max<uint64_t>(largest, 1));
-        if (status == SADCS_OK) status = m_arena.allocate(hist, radixHistWords(walk));
-        if (status == SADCS_OK) status = m_arena.allocate(aggregates, radixAggregateBytes(walk) / sizeof(Sum1) + 1);
+        // The histograms too go to the caller's buffer when it has room: every class list is live
+        // now, the peak of a device-resident build.
+        if (status == SADCS_OK) status = scratch(hist, radixHistWords(walk));
+        if (status == SADCS_OK) status = scratch(aggregates, radixAggregateBytes(walk) / sizeof(Sum1) + 1);
         if (status == SADCS_OK) status = scratch(walkKeys, std::max<uint64_t>(textWalk, 1));
         if (status == SADCS_OK) status = scratch(walkValues, std::max<uint64_t>(textWalk, 1));
         if (status != SADCS_OK || m_dry) {
