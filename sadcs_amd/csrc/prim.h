@@ -85,6 +85,10 @@ __global__ __launch_bounds__(kScanThreads) void scanReduceKernel(Producer produc
 }
 
 // Exclusive scan of count aggregates in place by one block, looping; the total goes to aggregates[count].
+// Each thread takes kAggregateItems consecutive aggregates per step, so a step covers 4096 (the
+// first ranking pass has about 115K tiles: 28 steps, where one aggregate per thread took 450).
+constexpr uint32_t kAggregateItems = 16;
+
 template <class VecT>
 __global__ __launch_bounds__(kScanThreads) void scanAggregatesKernel(VecT* __restrict__ aggregates, uint32_t count) {
     __shared__ VecT waveTotals[kScanWaves];
@@ -97,10 +101,19 @@ __global__ __launch_bounds__(kScanThreads) void scanAggregatesKernel(VecT* __res
     }
     __syncthreads();
 
-    for (uint32_t chunk = 0; chunk < count; chunk += kScanThreads) {
-        const uint32_t i = chunk + threadIdx.x;
-        const VecT x = i < count ? aggregates[i] : VecT::zero();
-        const VecT inclusive = VecT::waveInclusive(x);
+    for (uint32_t chunk = 0; chunk < count; chunk += kScanThreads * kAggregateItems) {
+        const uint32_t first = chunk + threadIdx.x * kAggregateItems;
+        VecT x[kAggregateItems];
+        VecT total = VecT::zero();
+#pragma unroll
+        for (uint32_t k = 0; k < kAggregateItems; k++) {
+            x[k] = first + k < count ? aggregates[first + k] : VecT::zero();
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kAggregateItems; k++) {
+            total = VecT::combine(total, x[k]);
+        }
+        const VecT inclusive = VecT::waveInclusive(total);
         if (lane == kWave - 1) {
             waveTotals[wave] = inclusive;
         }
@@ -109,9 +122,13 @@ __global__ __launch_bounds__(kScanThreads) void scanAggregatesKernel(VecT* __res
         for (uint32_t w = 0; w < wave; w++) {
             prefix = VecT::combine(prefix, waveTotals[w]);
         }
-        const VecT exclusive = VecT::combine(prefix, VecT::shiftUp(inclusive));
-        if (i < count) {
-            aggregates[i] = exclusive;
+        prefix = VecT::combine(prefix, VecT::shiftUp(inclusive));
+#pragma unroll
+        for (uint32_t k = 0; k < kAggregateItems; k++) {
+            if (first + k < count) {
+                aggregates[first + k] = prefix;
+            }
+            prefix = VecT::combine(prefix, x[k]);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
