@@ -142,8 +142,10 @@ __global__ __launch_bounds__(256) void makePassKeysKernel(RankParams rp, Texts t
 // before i: the same order as the stable radix sort, with each element reading only its group.
 // Unresolved groups are mostly pairs (a repeat and its source), so this replaces the several
 // digit passes of a 50- to 64-bit radix sort. A group of more than kGroupRankMax elements sets
-// *large and the caller sorts the pass by radix instead (the output is then ignored).
-constexpr uint32_t kGroupRankMax = 32;
+// *large and the caller sorts the pass by radix instead (the output is then ignored). Ordinals
+// do not decrease along the list, so one read kGroupRankMax away shows whether the group is
+// larger than that: elements of a huge group (periodic text) give up at once instead of scanning.
+constexpr uint32_t kGroupRankMax = 256;
 
 __global__ void groupRankSortKernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ values, uint32_t u, int subkeyBits, uint64_t* __restrict__ keysOut, uint32_t* __restrict__ valuesOut, uint32_t* __restrict__ large) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -152,6 +154,11 @@ __global__ void groupRankSortKernel(const uint64_t* __restrict__ keys, const uin
     }
     const uint64_t key = keys[i];
     const uint64_t ordinal = key >> subkeyBits;
+    if ((i >= kGroupRankMax && (keys[i - kGroupRankMax] >> subkeyBits) == ordinal) ||
+        (i + kGroupRankMax < u && (keys[i + kGroupRankMax] >> subkeyBits) == ordinal)) {
+        atomicOr(large, 1u);
+        return;
+    }
     uint32_t start = i;
     for (uint32_t step = 0; step < kGroupRankMax && start > 0 && (keys[start - 1] >> subkeyBits) == ordinal; step++) {
         start--;
