@@ -240,8 +240,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(texts would shard across GPUs as described in DESIGN.md, but no multi-GPU path is built)", flush=True)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
+              "(batches would shard by text across GPUs, one large text by a distributed sample sort; "
+              "no multi-GPU path is built)", flush=True)
         return 2
 
     import sadcs_amd

@@ -13,7 +13,7 @@
  * in the reference.
  *
  * Every construction runs on the GPU: --gpu is accepted and implied (the reference's CPU variant
- * is not part of this build). --cover selects the GPU's difference cover, 32 (default) or 128.
+ * is not part of this library). --cover selects the GPU's difference cover, 32 (default) or 128.
  * --both builds the suffix array and the BWT in one merge and writes <out> and <out>.bwt.
  * Sizes are MiB and rates MiB/s, as the reference prints them (main.cpp:175-177).
  */

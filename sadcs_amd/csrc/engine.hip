@@ -1767,7 +1767,7 @@ int buildWith(sadcs_ctx& ctx, const Layout& layout, const BuildRequest& rq, Text
 }
 
 int build(sadcs_ctx& ctx, const Layout& layout, const BuildRequest& rq, sadcs_stats* stats) {
-    (void)hipGetLastError();  // errors of earlier, unrelated calls are not this build's
+    (void)hipGetLastError();  // errors of earlier, unrelated calls are not this construction's
     const bool batch = !layout.blockSegment.empty();
     if (ctx.params.cover == 128) {
         return batch ? buildWith<128, TextBatch>(ctx, layout, rq, TextBatch{}, stats)

@@ -100,7 +100,7 @@ def test_check_reports_the_first_failure(selftest_text, tmp_path, mode, flag):
         want = reference_check(text, bad, mode)
         if name == "duplicate entry" and mode == 0:
             # The reference's fast check lets OpenMP threads race for the inverse slot of a
-            # duplicated value, so it names either copy; this build names the first.
+            # duplicated value, so it names either copy; this library names the first.
             assert want in (500, 1500)
         elif want is not None:
             assert got == want, (name, out)
