@@ -144,3 +144,21 @@ def test_check_of_a_device_build(gpu):
     want = oracle.check_sa(text.tobytes(), bad)
     d_bad = gpu.DeviceBuffer.from_host(bad)
     assert gpu.gpu_check_device(d_text.ptr, n, d_bad.ptr) == (3, want)
+
+
+def test_largest_text_passes_the_device_check(gpu):
+    """The largest text the library takes (2^31 - 1025 bytes: positions near 2^31 in 32-bit
+    entries, 8 GiB suffix array): SA + BWT on the device, every row checked there."""
+    from sadcs_amd import workloads
+    n = (1 << 31) - 1025
+    text = workloads.markov(n, 99, padding=64)
+    d_text = gpu.DeviceBuffer.from_host(text)
+    del text
+    d_sa = gpu.DeviceBuffer(4 * n)
+    d_bwt = gpu.DeviceBuffer(n)
+    with gpu.GpuContext(n) as ctx:
+        stats = gpu.GpuStats()
+        primary = gpu.gpu_build_device(ctx, d_text.ptr, n, d_sa.ptr, d_bwt.ptr, stats)
+    print(f"n={n}: device {stats.device_seconds * 1e3:.1f} ms, peak {stats.device_bytes / n:.2f} B/byte", flush=True)
+    assert stats.device_bytes <= 10.1 * n
+    assert gpu.gpu_check_device(d_text.ptr, n, d_sa.ptr, d_bwt.ptr, primary) == (0, -1)
