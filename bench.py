@@ -74,7 +74,32 @@ def cpu_baseline(text_prefix):
             "sample": f"first {n >> 20} MiB of the same text; reference CPU build (cover 128), SA {t1 - t0:.2f}s + BWT {t2 - t1:.2f}s"}
 
 
-def batched_measurement(count, steps, tune=True):
+def cpu_texts_baseline(texts, what):
+    """The reference CPU build (all host threads) over a list of texts one after another, SA then
+    BWT of each: the sequential CPU path for the batched and served workloads."""
+    import numpy as np
+
+    import oracle
+    if not oracle.ref_available():
+        return None
+    lib = oracle.ref_lib()
+    total = 0
+    t0 = time.perf_counter()
+    for text in texts:
+        n = int(text.size)
+        padded = np.zeros(n + oracle.TEXT_PADDING, dtype=np.uint8)
+        padded[:n] = text
+        sa = np.empty(n, dtype=np.int32)
+        assert lib.ref_build_sa(padded.ctypes.data, n, sa.ctypes.data, 128) == 0
+        out = np.empty(n, dtype=np.uint8)
+        lib.ref_build_bwt(padded.ctypes.data, n, out.ctypes.data, sa.ctypes.data, 128)
+        total += n
+    seconds = time.perf_counter() - t0
+    return {"value": round(total / 1e6 / seconds, 2), "unit": "MB/s", "cores": int(lib.ref_max_threads()), "kind": "reference",
+            "sample": f"{what}: {len(texts)} texts, {total} bytes, SA + BWT each, one after another, {seconds:.2f}s"}
+
+
+def batched_measurement(count, steps, tune=True, cpu_texts=0):
     """Batched construction of `count` mixed texts against one-at-a-time device builds."""
     import numpy as np
 
@@ -150,14 +175,15 @@ def batched_measurement(count, steps, tune=True):
         same = same and np.array_equal(a.to_host(np.uint8), b.to_host(np.uint8))
     for buf in (d_texts, d_seq, d_sa, d_bwt, d_sa2, d_bwt2):
         buf.free()
+    cpu = cpu_texts_baseline(texts[:cpu_texts], f"the first {cpu_texts} of the {count} texts") if cpu_texts else None
     return {"texts": count, "bytes": total, "mb_per_s": round(total / batch_seconds / 1e6, 1),
             "ms": round(batch_seconds * 1e3, 2), "sequential_mb_per_s": round(total / seq_seconds / 1e6, 1),
             "sequential_ms": round(seq_seconds * 1e3, 1), "speedup_vs_sequential": round(seq_seconds / batch_seconds, 2),
             "outputs_equal_sequential": same, "resident": "texts in HBM, outputs in HBM",
-            "merge_params": tuned or "defaults"}
+            "merge_params": tuned or "defaults", "cpu_baseline": cpu}
 
 
-def served_measurement(requests, factors=(0.5, 0.8, 1.1, None)):
+def served_measurement(requests, factors=(0.5, 0.8, 1.1, None), cpu=False):
     """Poisson request streams through the serving loop, against one-at-a-time host builds. A factor
     of None submits every request at once: the server's capacity with full batches."""
     import threading
@@ -219,7 +245,8 @@ def served_measurement(requests, factors=(0.5, 0.8, 1.1, None)):
                       "mean_batch": round(float(np.mean([r[5] for r in results])), 2), "batches": counters["batches"]})
         print(f"# served at {factor}x: {rates[-1]}", file=sys.stderr, flush=True)
     return {"requests": requests, "bytes": total, "sequential_mb_per_s": round(total / seq_seconds / 1e6, 1),
-            "sequential_requests_per_s": round(capacity, 1), "rates": rates, "io": "host memory in and out (PCIe)"}
+            "sequential_requests_per_s": round(capacity, 1), "rates": rates, "io": "host memory in and out (PCIe)",
+            "cpu_baseline": cpu_texts_baseline(texts, "every request") if cpu else None}
 
 
 def main():
@@ -324,8 +351,9 @@ def main():
     ctx.close()
     for buf in (d_text, d_sa, d_bwt):
         buf.free()
-    batched = batched_measurement(args.batch_texts, 3, tune=args.tune_mib > 0) if args.batch_texts else None
-    served = served_measurement(args.serve_requests) if args.serve_requests else None
+    batched = batched_measurement(args.batch_texts, 3, tune=args.tune_mib > 0,
+                                  cpu_texts=0 if args.no_cpu else min(512, args.batch_texts)) if args.batch_texts else None
+    served = served_measurement(args.serve_requests, cpu=not args.no_cpu) if args.serve_requests else None
     ranking, classes, merge = (sum(p[i] for p in phases) / len(phases) for i in range(3))
     line = {
         "metric": METRIC,
