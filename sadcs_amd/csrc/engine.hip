@@ -425,10 +425,28 @@ struct Job {
     bool textScratch = false;               // the text is this call's upload: free to overwrite late in the merge
     uint32_t maxLength = 0;
     uint64_t totalLength = 0;               // positions in texts (output rows)
+    int32_t* hostSA = nullptr;              // host calls: where the outputs go; the merge
+    uint8_t* hostBWT = nullptr;             //   downloads them as it finishes slices of rows
+    bool downloaded = false;                //   and sets this when it has
 };
+
+// The outputs of a top-level merge in slices, for downloadSlices: rows[0] = 0 < ... <
+// rows[slices] = totalRows (host copies of the slices' first rows).
+struct OutputSlices {
+    uint32_t slices = 0;
+    const uint32_t* rows = nullptr;
+    uint64_t totalRows = 0;
+    uint8_t* hostSA = nullptr;
+    const uint8_t* deviceSA = nullptr;
+    uint8_t* hostBWT = nullptr;
+    const uint8_t* deviceBWT = nullptr;
+};
+
+int downloadSlices(sadcs_ctx& ctx, const OutputSlices& o);
 
 struct PhaseTimes {
     double ranking = 0, classes = 0, merge = 0, bwt = 0;
+    double download = 0;  // host seconds of a download that overlapped the merge, past its end
     double mergeKernel = 0;
     uint64_t mergeElements = 0;
     int characterPasses = 0, doublingRounds = 0;
@@ -452,11 +470,25 @@ struct sadcs_ctx {
     hipEvent_t stagingDone[2] = {};
     hipEvent_t kernelTimer[2] = {};    // around the top-level merge kernel
     int sms = 0;
+    // Downloads that overlap the top-level merge: its launches over consecutive row slices each
+    // record sliceDone; copyStream copies finished slices to the host while later ones merge.
+    static constexpr int kSlices = 16;
+    hipStream_t copyStream = nullptr;  // null: no overlap (download after the merge)
+    hipEvent_t sliceDone[kSlices] = {};
+    hipEvent_t mergeBegin = nullptr;
 };
 
 namespace sadcs {
 
 constexpr size_t kStagingChunk = size_t(32) << 20;
+// Pinned readback words: [0, 264) the text's byte presence (at the start of ranking), [0, 16)
+// counters, [16, 1016) the first rows of a batch (after the build), [kSliceRowWords, +kSlices + 1)
+// the row bounds of the top-level merge's slices.
+constexpr size_t kHostWords = 2048;
+constexpr size_t kSliceRowWords = 1024;
+// The top-level merge of a host call is launched in slices of about this many output rows (at
+// most sadcs_ctx::kSlices), so the download of finished rows overlaps the rest of the merge.
+constexpr uint64_t kSliceRows = uint64_t(1) << 24;
 constexpr int64_t kMemoryReserve = int64_t(256) << 20;
 constexpr size_t kTextPadding = 64;
 
@@ -552,6 +584,13 @@ class Pipeline {
         SADCS_TRY(sync());
         times.merge = secondsSince(start);
         times.mergeBytes = m_arena.highWater();
+        if (m_job.downloaded) {
+            // The merge phase ends with its last slice; the host time past that was the download's.
+            float ms = 0;
+            SADCS_HIP(hipEventElapsedTime(&ms, m_ctx.mergeBegin, m_ctx.sliceDone[m_slices - 1]));
+            times.download = std::max(0.0, times.merge - ms * 1e-3);
+            times.merge = ms * 1e-3;
+        }
         return SADCS_OK;
     }
 
@@ -938,7 +977,7 @@ class Pipeline {
     }
 
     template <int kFlags>
-    int launchTiles(const ListSet& lists, const uint32_t* rows, uint32_t tiles, const TileOut& out, uint32_t* errors) {
+    int launchTiles(const ListSet& lists, const uint32_t* rows, uint32_t tiles, const TileOut& out, uint32_t* errors, uint32_t slices = 1) {
         const uint32_t capacity = mergeCapacity<V>(m_g, m_k);
         const size_t shared = mergeSharedBytes<V, kBatch>(capacity);
         if (m_dry) {
@@ -950,8 +989,17 @@ class Pipeline {
         }
         auto kernel = mergeTilesKernel<V, Texts, kFlags>;
         SADCS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shared)));
-        kernel<<<tiles, kMergeThreads, shared, m_stream>>>(m_ctxMerge, m_texts, lists, rows, capacity, out, errors);
-        SADCS_HIP(hipGetLastError());
+        // Slices (a host call's top level): consecutive tile ranges, each followed by its event.
+        const uint64_t stride = static_cast<uint64_t>(lists.count) + 1;
+        for (uint32_t s = 0; s < slices; s++) {
+            const uint32_t first = static_cast<uint32_t>(static_cast<uint64_t>(tiles) * s / slices);
+            const uint32_t last = static_cast<uint32_t>(static_cast<uint64_t>(tiles) * (s + 1) / slices);
+            kernel<<<last - first, kMergeThreads, shared, m_stream>>>(m_ctxMerge, m_texts, lists, rows + first * stride, capacity, out, errors);
+            SADCS_HIP(hipGetLastError());
+            if (slices > 1) {
+                SADCS_HIP(hipEventRecord(m_ctx.sliceDone[s], m_stream));
+            }
+        }
         if (kTop && m_ctx.kernelTimer[0]) {
             SADCS_HIP(hipEventRecord(m_ctx.kernelTimer[1], m_stream));
             m_timedMerge = true;
@@ -961,6 +1009,7 @@ class Pipeline {
     }
 
     bool m_timedMerge = false;
+    uint32_t m_slices = 1;  // launches of the top level (more than one: the download overlapped them)
     uint4* m_lines = nullptr;
     bool m_bwtInText = false;
 
@@ -1075,8 +1124,12 @@ class Pipeline {
         uint32_t* errors = nullptr;
         uint32_t* firstRows = m_job.firstRows;
         bool ownFirstRows = false;
-        SADCS_TRY(m_arena.allocate(errors, 1));
+        // errors[0]: the tiles' error flags; errors[1..]: the slices' first rows.
+        SADCS_TRY(m_arena.allocate(errors, 2 + sadcs_ctx::kSlices));
         fillWords(errors, 1, 0);
+        if (m_ctx.mergeBegin && !m_dry) {
+            SADCS_HIP(hipEventRecord(m_ctx.mergeBegin, m_stream));
+        }
 
         if (m_job.wantBWT && !firstRows) {
             SADCS_TRY(m_arena.allocate(firstRows, m_segments));
@@ -1092,6 +1145,21 @@ class Pipeline {
         uint32_t* rows = nullptr;
         uint32_t tiles = 0;
         status = levelRows(lists, rows, tiles, errors);
+
+        // A host call downloads the top level's output rows slice by slice while later slices
+        // merge (a host-side choice: the kernels and their results are those of one launch).
+        uint32_t slices = 1;
+        if (status == SADCS_OK && m_ctx.copyStream && (m_job.hostSA || m_job.hostBWT)) {
+            slices = static_cast<uint32_t>(std::min<uint64_t>({static_cast<uint64_t>(sadcs_ctx::kSlices),
+                                                               (m_job.totalLength + kSliceRows - 1) / kSliceRows, tiles}));
+        }
+        auto launchSliceRows = [&]() -> int {
+            if (slices > 1 && !m_dry) {
+                sliceRowsKernel<<<1, 64, 0, m_stream>>>(rows, static_cast<uint32_t>(lists.count), tiles, slices, errors + 1);
+                SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords + kSliceRowWords, errors + 1, (slices + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
+            }
+            return SADCS_OK;
+        };
 
         TileOut out{};
         out.firstSuffixRow = firstRows;
@@ -1137,8 +1205,11 @@ class Pipeline {
                 }
             }
             if (status == SADCS_OK) {
-                status = m_job.wantBWT ? launchTiles<kOutSA | kOutBWT | kInPlace>(mergeLists, rows, tiles, out, errors)
-                                       : launchTiles<kOutSA | kInPlace>(mergeLists, rows, tiles, out, errors);
+                status = launchSliceRows();
+            }
+            if (status == SADCS_OK) {
+                status = m_job.wantBWT ? launchTiles<kOutSA | kOutBWT | kInPlace>(mergeLists, rows, tiles, out, errors, slices)
+                                       : launchTiles<kOutSA | kInPlace>(mergeLists, rows, tiles, out, errors, slices);
             }
         } else if (status == SADCS_OK && m_job.wantBWT) {
             out.bwt = m_job.bwt;
@@ -1150,8 +1221,31 @@ class Pipeline {
                 firstBwtBytesKernel<<<blocksFor(m_segments), 256, 0, m_stream>>>(m_job.text, m_job.deviceSegments, m_segments, out.bwt);
             }
             if (status == SADCS_OK) {
-                status = launchTiles<kOutBWT>(mergeLists, rows, tiles, out, errors);
+                status = launchSliceRows();
             }
+            if (status == SADCS_OK) {
+                status = launchTiles<kOutBWT>(mergeLists, rows, tiles, out, errors, slices);
+            }
+        }
+
+        if (status == SADCS_OK && slices > 1 && !m_dry) {
+            // The slices' rows were read back before the first slice ran.
+            SADCS_HIP(hipEventSynchronize(m_ctx.sliceDone[0]));
+            OutputSlices o;
+            o.slices = slices;
+            o.rows = m_ctx.hostWords + kSliceRowWords;
+            o.totalRows = m_job.totalLength;
+            if (m_job.wantSA && m_job.hostSA) {
+                o.hostSA = reinterpret_cast<uint8_t*>(m_job.hostSA);
+                o.deviceSA = reinterpret_cast<const uint8_t*>(out.sa);
+            }
+            if (m_job.wantBWT && m_job.hostBWT) {
+                o.hostBWT = m_job.hostBWT;
+                o.deviceBWT = out.bwt;
+            }
+            status = downloadSlices(m_ctx, o);
+            m_job.downloaded = status == SADCS_OK;
+            m_slices = slices;
         }
 
         if (status == SADCS_OK && !m_dry) {
@@ -1527,6 +1621,97 @@ int download(sadcs_ctx& ctx, void* host, const void* device, size_t bytes) {
     return SADCS_OK;
 }
 
+void releaseOverlap(sadcs_ctx& ctx) {
+    if (ctx.copyStream) {
+        (void)hipStreamSynchronize(ctx.copyStream);
+        (void)hipStreamDestroy(ctx.copyStream);
+        ctx.copyStream = nullptr;
+    }
+    for (hipEvent_t& e : ctx.sliceDone) {
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    }
+    if (ctx.mergeBegin) {
+        (void)hipEventDestroy(ctx.mergeBegin);
+        ctx.mergeBegin = nullptr;
+    }
+}
+
+// Downloads the outputs of a top-level merge launched in slices (OutputSlices) while it runs: after
+// slice s the SA rows [rows[s], rows[s + 1]) and the BWT bytes [rows[s], rows[s + 1]) are final (BWT
+// byte j comes from row j - 1 or row j, byte 0 of each text from before the merge), so the copy
+// stream waits for that slice's event and copies them, through the staging chunks for pageable
+// memory and directly into pinned memory. Returns with the data on the host.
+int downloadSlices(sadcs_ctx& ctx, const OutputSlices& o) {
+    struct Chunk {
+        uint8_t* host;
+        const uint8_t* device;
+        size_t bytes;
+        uint32_t slice;
+        int buffer;  // staging chunk, or -1: a direct copy into pinned memory
+    };
+    std::vector<Chunk> chunks;
+    const bool pinnedSA = o.hostSA && isPinned(o.hostSA);
+    const bool pinnedBWT = o.hostBWT && isPinned(o.hostBWT);
+    uint64_t staged = 0;
+    auto add = [&](uint8_t* host, const uint8_t* device, size_t bytes, uint32_t slice, bool pinned) {
+        for (size_t offset = 0; offset < bytes; offset += kStagingChunk) {
+            const size_t length = pinned ? bytes : std::min(kStagingChunk, bytes - offset);
+            chunks.push_back(Chunk{host + offset, device + offset, length, slice, pinned ? -1 : static_cast<int>(staged++ & 1)});
+            if (pinned) {
+                break;
+            }
+        }
+    };
+    for (uint32_t s = 0; s < o.slices; s++) {
+        const uint64_t a = o.rows[s], b = o.rows[s + 1];
+        if (a > b || b > o.totalRows || (s + 1 == o.slices && b != o.totalRows)) {
+            setError("merge slices: rows %llu..%llu of %llu (internal error)", static_cast<unsigned long long>(a),
+                     static_cast<unsigned long long>(b), static_cast<unsigned long long>(o.totalRows));
+            return SADCS_FAILED;
+        }
+        if (o.hostSA) {
+            add(o.hostSA + 4 * a, o.deviceSA + 4 * a, 4 * (b - a), s, pinnedSA);
+        }
+        if (o.hostBWT) {
+            add(o.hostBWT + a, o.deviceBWT + a, b - a, s, pinnedBWT);
+        }
+    }
+    uint32_t waited = UINT32_MAX;
+    auto issue = [&](size_t k) -> int {
+        const Chunk& c = chunks[k];
+        if (c.slice != waited) {
+            SADCS_HIP(hipStreamWaitEvent(ctx.copyStream, ctx.sliceDone[c.slice], 0));
+            waited = c.slice;
+        }
+        uint8_t* target = c.buffer < 0 ? c.host : ctx.staging[c.buffer];
+        SADCS_HIP(hipMemcpyAsync(target, c.device, c.bytes, hipMemcpyDeviceToHost, ctx.copyStream));
+        if (c.buffer >= 0) {
+            SADCS_HIP(hipEventRecord(ctx.stagingDone[c.buffer], ctx.copyStream));
+        }
+        return SADCS_OK;
+    };
+    // Staged chunks alternate between the two buffers; a buffer is refilled (chunk k + 1) only
+    // after the host has copied out everything before chunk k.
+    if (!chunks.empty()) {
+        SADCS_TRY(issue(0));
+    }
+    for (size_t k = 0; k < chunks.size(); k++) {
+        if (k + 1 < chunks.size()) {
+            SADCS_TRY(issue(k + 1));
+        }
+        const Chunk& c = chunks[k];
+        if (c.buffer >= 0) {
+            SADCS_HIP(hipEventSynchronize(ctx.stagingDone[c.buffer]));
+            HostCopier::instance().copy(c.host, ctx.staging[c.buffer], c.bytes);
+        }
+    }
+    SADCS_HIP(hipStreamSynchronize(ctx.copyStream));
+    return SADCS_OK;
+}
+
 int createContext(size_t arenaBytes, const Params& params, int32_t capacity, sadcs_ctx*& out) {
     out = nullptr;
     sadcs_device device;
@@ -1557,7 +1742,7 @@ int createContext(size_t arenaBytes, const Params& params, int32_t capacity, sad
             return statusOf(e);
         }
     }
-    if (hipHostMalloc(reinterpret_cast<void**>(&ctx->hostWords), 4096, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&ctx->hostWords), kHostWords * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         setError("hipHostMalloc of readback words failed");
         (void)hipFree(ctx->arenaBase);
@@ -1586,6 +1771,16 @@ int createContext(size_t arenaBytes, const Params& params, int32_t capacity, sad
             break;
         }
     }
+    // Without its stream or any of its events the context downloads after the merge.
+    bool overlap = ctx->staging[0] != nullptr && hipStreamCreateWithFlags(&ctx->copyStream, hipStreamNonBlocking) == hipSuccess;
+    overlap = overlap && hipEventCreate(&ctx->mergeBegin) == hipSuccess;
+    for (int i = 0; i < sadcs_ctx::kSlices && overlap; i++) {
+        overlap = hipEventCreate(&ctx->sliceDone[i]) == hipSuccess;
+    }
+    if (!overlap) {
+        (void)hipGetLastError();
+        releaseOverlap(*ctx);
+    }
     out = ctx.release();
     return SADCS_OK;
 }
@@ -1595,6 +1790,7 @@ void destroyContext(sadcs_ctx* ctx) {
         return;
     }
     (void)hipStreamSynchronize(ctx->stream);
+    releaseOverlap(*ctx);
     for (int i = 0; i < 2; i++) {
         if (ctx->kernelTimer[i]) {
             (void)hipEventDestroy(ctx->kernelTimer[i]);
@@ -1711,6 +1907,10 @@ int buildWith(sadcs_ctx& ctx, const Layout& layout, const BuildRequest& rq, Text
     job.textScratch = !deviceCall;
     job.maxLength = layout.maxLength;
     job.totalLength = layout.totalLength;
+    if (!deviceCall) {
+        job.hostSA = job.wantSA ? rq.hostSA : nullptr;
+        job.hostBWT = job.wantBWT ? rq.hostBWT : nullptr;
+    }
 
     PhaseTimes times;
     uint32_t* outSA = nullptr;
@@ -1750,7 +1950,7 @@ int buildWith(sadcs_ctx& ctx, const Layout& layout, const BuildRequest& rq, Text
     if (job.rankOnly && rq.hostRanks) {
         SADCS_TRY(download(ctx, rq.hostRanks, keep->ranks(), sizeof(uint32_t) * samplesOf<V>(layout.span)));
     }
-    if (!deviceCall) {
+    if (!deviceCall && !job.downloaded) {
         if (job.wantSA && rq.hostSA) {
             SADCS_TRY(download(ctx, rq.hostSA, outSA, static_cast<size_t>(layout.totalLength) * 4));
         }
@@ -1760,7 +1960,7 @@ int buildWith(sadcs_ctx& ctx, const Layout& layout, const BuildRequest& rq, Text
     }
     SADCS_HIP(hipStreamSynchronize(ctx.stream));
     if (stats) {
-        stats->download_seconds = secondsSince(downloadStart);
+        stats->download_seconds = secondsSince(downloadStart) + times.download;
         (void)wallStart;
     }
     return SADCS_OK;

@@ -499,6 +499,17 @@ __device__ __forceinline__ void tileLayout(const ListSet& lists, const uint32_t*
     }
 }
 
+// The first output row of every slice of a level's tiles, and the total after them: slice s starts
+// at tile s * tiles / slices, and row tile * (lists + 1) + lists of the row table is a tile's first
+// output row (tile = tiles: the total). One workgroup; slices < 64.
+__global__ void sliceRowsKernel(const uint32_t* __restrict__ rows, uint32_t lists, uint32_t tiles, uint32_t slices, uint32_t* __restrict__ out) {
+    const uint32_t s = threadIdx.x;
+    if (s <= slices) {
+        const uint64_t tile = static_cast<uint64_t>(tiles) * s / slices;
+        out[s] = rows[tile * (lists + 1) + lists];
+    }
+}
+
 // ---- tile-major copy ----------------------------------------------------------------------
 
 // Copies every tile's parts of the lists to its output range, list after list.
