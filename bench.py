@@ -14,12 +14,15 @@ Printed as one JSON line, with:
     of known size, profiles/mergeTilesKernel_c3_1GiB_traffic.md);
   e2e_hbm_fraction: the whole step's algorithmic bytes (127.94 bytes per input byte for fused
     SA + BWT) over the step time, against 8 TB/s;
+  host_call: the same 1 GiB text through the host-memory API (pageable text in, SA and BWT out,
+    PCIe included; the download overlaps the end of the merge), best of 3 calls, outputs compared
+    with the device-resident build's - a secondary figure, never value;
   cpu_baseline: the reference's CPU construction (oracle/_ref, OpenMP on all host cores) timed
     on a 256 MiB prefix of the same generator's text, SA and BWT built separately as the reference
     does, MB/s = prefix bytes / (SA seconds + BWT seconds);
   batched: 4096 texts of log-uniform length 1 KiB - 4 MiB (even: DNA-like, odd: Markov), all
     resident in HBM, built in one batched construction, against the same texts built one at a
-    time by the single-text device path; per-text outputs of both compared on a sample;
+    time by the single-text device path; every text's outputs of both compared;
   served: requests (lognormal lengths, median 256 KiB, sigma 1.5, capped at 64 MiB; 45% ACGT,
     45% protein-like, 10% periodic) arriving as a Poisson stream at 0.5, 0.8 and 1.1 times the
     measured one-at-a-time capacity, through the continuous-batching server (host memory in and
@@ -249,6 +252,46 @@ def served_measurement(requests, factors=(0.5, 0.8, 1.1, None), cpu=False):
             "cpu_baseline": cpu_texts_baseline(texts, "every request") if cpu else None}
 
 
+def host_call_measurement(ctx, text, d_sa, d_bwt, primary, runs):
+    """The same text through the host-memory API (gpuSuffixArray + gpuBwt in one call): text uploaded
+    from pageable memory, SA and BWT returned to fresh pageable arrays, PCIe included. Best of runs;
+    the outputs must equal the device-resident build's."""
+    import ctypes
+    import numpy as np
+    import sadcs_amd
+    n = text.size
+    best = None
+    for r in range(runs):
+        # The C ABI call alone, into fresh arrays (the Python wrapper's copy of the BWT into a
+        # bytes object is not the library's).
+        stats = sadcs_amd.GpuStats()
+        sa = np.empty(n, dtype=np.int32)
+        bwt = np.empty(n, dtype=np.uint8)
+        got = ctypes.c_int32(-1)
+        start = time.perf_counter()
+        status = sadcs_amd.lib.sadcs_hip_build(ctx.handle, text.ctypes.data, n, sa.ctypes.data, bwt.ctypes.data,
+                                               ctypes.byref(got), ctypes.byref(stats))
+        wall = time.perf_counter() - start
+        assert status == 0, sadcs_amd.gpu_last_error()
+        got_primary = got.value
+        print(f"# host call {r}: {wall * 1e3:.1f} ms (upload {stats.upload_seconds * 1e3:.1f}, device "
+              f"{stats.device_seconds * 1e3:.1f}, download after the merge {stats.download_seconds * 1e3:.1f})",
+              file=sys.stderr, flush=True)
+        if best is None or wall < best[0]:
+            best = (wall, stats.upload_seconds, stats.device_seconds, stats.download_seconds)
+        if r + 1 < runs:
+            del sa, bwt
+    same = bool(got_primary == primary and np.array_equal(sa, d_sa.to_host(np.int32))
+                and np.array_equal(bwt, d_bwt.to_host(np.uint8)))
+    assert same, "host call outputs differ from the device-resident build's"
+    wall, upload, device, download = best
+    return {"value": round(n / wall / 1e6, 1), "unit": "MB/s", "ms": round(wall * 1e3, 1),
+            "upload_ms": round(upload * 1e3, 1), "device_ms": round(device * 1e3, 1),
+            "download_after_merge_ms": round(download * 1e3, 1), "runs": runs,
+            "io": "pageable host memory in and out over PCIe; the download overlaps the top-level merge",
+            "outputs_equal_device_build": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +305,7 @@ def main():
                     help="autotune the merge configuration on a prefix of this many MiB before timing (0: defaults)")
     ap.add_argument("--batch-texts", type=int, default=4096, help="texts in the batched measurement (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=300, help="requests per offered rate (0: skip)")
+    ap.add_argument("--host-runs", type=int, default=3, help="host-memory calls of the same text (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device to device three times (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -343,6 +387,8 @@ def main():
     assert verdict == (0, -1), f"output check failed: {sadcs_amd.CHECK_NAMES[verdict[0]]} at index {verdict[1]}"
     print(f"# device check of all {n} rows passed in {check_seconds:.2f}s", file=sys.stderr, flush=True)
 
+    host_call = host_call_measurement(ctx, text[:n], d_sa, d_bwt, primary, args.host_runs) if args.host_runs else None
+
     baseline = None
     if not args.no_cpu:
         sample = min(n, args.cpu_sample_mib << 20)
@@ -381,6 +427,7 @@ def main():
         "cpu_baseline": baseline,
         "output_check": {"kind": "device, every row: permutation, neighbour order, BWT and primary",
                          "result": "pass", "seconds": round(check_seconds, 3)},
+        "host_call": host_call,
         "batched": batched,
         "served": served,
     }

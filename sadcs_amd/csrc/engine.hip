@@ -681,9 +681,11 @@ class Pipeline {
         MarkVec* markAggregates = nullptr;
         uint32_t* counts = nullptr;
         uint32_t* byteCounts = nullptr;
+        uint8_t* digits = nullptr;  // the radix passes' next digits, one byte per sample
 
         int status = SADCS_OK;
         auto cleanup = [&]() {
+            unscratch(digits);
             m_arena.release(byteCounts);
             m_arena.release(counts);
             m_arena.release(markAggregates);
@@ -706,6 +708,7 @@ class Pipeline {
         if (status == SADCS_OK) status = m_arena.allocate(markAggregates, scanScratchBytes<MarkVec, kMarkItems>(M) / sizeof(MarkVec) + 1);
         if (status == SADCS_OK) status = m_arena.allocate(counts, 4);
         if (status == SADCS_OK) status = m_arena.allocate(byteCounts, 8);
+        if (status == SADCS_OK) status = scratch(digits, std::max<uint32_t>(M, 1));
         if (status != SADCS_OK || m_dry || M == 0) {
             cleanup();
             return status;
@@ -752,6 +755,7 @@ class Pipeline {
         sb.values[1] = values[1];
         sb.hist = hist;
         sb.aggregates = sortAggregates;
+        sb.digits = digits;
 
         uint32_t u = M;
         uint32_t groups = m_segments;

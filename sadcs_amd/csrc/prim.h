@@ -313,12 +313,56 @@ __global__ __launch_bounds__(kSortThreads) void radixUpsweepKernel(Source source
         keys[j] = 0;
         valids[j] = source(clampIndex(i, count), keys[j], value) && i < count;
     }
+    // Counting needs no ranks: one LDS atomic add per item into the wave's own histogram (lanes
+    // with equal digits serialise in the LDS; matching them first cost more VALU time).
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
-        const uint32_t digit = digitOf(keys[j], shift, mask);
-        const uint64_t peers = matchDigit<kRadixBits>(digit, valids[j]);
-        if (valids[j] && lane == static_cast<uint32_t>(__builtin_ctzll(peers))) {
-            counts[wave][digit] += static_cast<uint32_t>(__builtin_popcountll(peers));
+        if (valids[j]) {
+            atomicAdd(&counts[wave][digitOf(keys[j], shift, mask)], 1u);
+        }
+    }
+    __syncthreads();
+
+    for (uint32_t d = threadIdx.x; d < kRadixBins; d += kSortThreads) {
+        uint32_t total = 0;
+        for (int w = 0; w < kSortWaves; w++) {
+            total += counts[w][d];
+        }
+        hist[static_cast<uint64_t>(d) * tiles + tile] = total;
+    }
+}
+
+// The histograms of a pass whose digits the previous pass wrote out, one byte per item in the
+// items' order (radixDownsweepKernel with kDigits): a tile reads 1 byte per item instead of its
+// keys. hist as radixUpsweepKernel writes it.
+__global__ __launch_bounds__(kSortThreads) void radixDigitUpsweepKernel(const uint8_t* __restrict__ digits, uint32_t count, uint32_t tiles, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t counts[kSortWaves][kRadixBins];
+    const uint32_t wave = threadIdx.x / kWave;
+
+    for (uint32_t x = threadIdx.x; x < kSortWaves * kRadixBins; x += kSortThreads) {
+        (&counts[0][0])[x] = 0;
+    }
+    __syncthreads();
+
+    // Thread x counts items [first, first + 8) of the tile: one 8-byte load (8-aligned: the buffer
+    // is, and so are tile and thread offsets) unless the tile ends inside them.
+    const uint32_t tile = xcdTile(blockIdx.x, tiles);
+    const uint64_t first = static_cast<uint64_t>(tile) * kSortTile + static_cast<uint64_t>(threadIdx.x) * kSortItems;
+    static_assert(kSortItems == 8, "one 8-byte load per thread");
+    uint64_t packed = 0;
+    if (first + kSortItems <= count) {
+        packed = *reinterpret_cast<const uint64_t*>(digits + first);
+    } else {
+        for (int j = 0; j < kSortItems; j++) {
+            if (first + j < count) {
+                packed |= static_cast<uint64_t>(digits[first + j]) << (8 * j);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        if (first + j < count) {
+            atomicAdd(&counts[wave][static_cast<uint32_t>(packed >> (8 * j)) & 0xffu], 1u);
         }
     }
     __syncthreads();
@@ -333,9 +377,12 @@ __global__ __launch_bounds__(kSortThreads) void radixUpsweepKernel(Source source
 }
 
 // offsets = exclusive sum of hist (digit-major): the first output index of each digit of each tile.
-// Writes beyond `limit` outputs are dropped (a source that keeps more items than planned).
-template <class K, class Source>
-__global__ __launch_bounds__(kSortThreads) void radixDownsweepKernel(Source source, uint32_t count, int shift, uint32_t mask, uint32_t tiles, const uint32_t* __restrict__ offsets, K* __restrict__ keysOut, uint32_t* __restrict__ valuesOut, uint32_t limit) {
+// Writes beyond `limit` outputs are dropped (a source that keeps more items than planned). With
+// kDigits, every output item's next digit ((key >> nextShift) & nextMask) goes to digitsOut at its
+// output index, for radixDigitUpsweepKernel.
+template <class K, class Source, bool kDigits = false>
+__global__ __launch_bounds__(kSortThreads) void radixDownsweepKernel(Source source, uint32_t count, int shift, uint32_t mask, uint32_t tiles, const uint32_t* __restrict__ offsets, K* __restrict__ keysOut, uint32_t* __restrict__ valuesOut, uint32_t limit,
+                                                                     uint8_t* __restrict__ digitsOut = nullptr, int nextShift = 0, uint32_t nextMask = 0) {
     __shared__ uint32_t waveCounts[kSortWaves][kRadixBins];
     __shared__ uint32_t digitStart[kRadixBins];
     __shared__ uint32_t globalStart[kRadixBins];
@@ -429,6 +476,9 @@ __global__ __launch_bounds__(kSortThreads) void radixDownsweepKernel(Source sour
         if (target < limit) {
             keysOut[target] = key;
             valuesOut[target] = sortedValues[x];
+            if constexpr (kDigits) {
+                digitsOut[target] = static_cast<uint8_t>(digitOf(key, nextShift, nextMask));
+            }
         }
     }
 }
@@ -442,6 +492,7 @@ struct SortBuffers {
     int current = 0;        // which buffer holds the data
     uint32_t* hist = nullptr;
     Sum1* aggregates = nullptr;
+    uint8_t* digits = nullptr;  // optional, one byte per item: radixSort's passes hand on their next digit
 };
 
 inline size_t radixHistWords(uint64_t capacity) {
@@ -471,9 +522,33 @@ inline void radixSort(hipStream_t stream, SortBuffers<K>& b, uint32_t count, int
     if (count <= 1 || endBit <= beginBit) {
         return;
     }
+    const uint32_t tiles = std::max<uint32_t>(1, ceilDiv(count, kSortTile));
     for (int shift = beginBit; shift < endBit; shift += kRadixBits) {
         const int width = std::min(kRadixBits, endBit - shift);
-        radixPass<K>(stream, b, ArraySource<K>{b.keys[b.current], b.values[b.current]}, count, shift, width);
+        const ArraySource<K> source{b.keys[b.current], b.values[b.current]};
+        const int nextShift = shift + kRadixBits;
+        if (!b.digits || (shift == beginBit && nextShift >= endBit)) {
+            radixPass<K>(stream, b, source, count, shift, width);
+            continue;
+        }
+        // With a digit buffer: every pass after the first counts the bytes the one before wrote.
+        // The buffer is read by this pass's histograms before its scatter overwrites it.
+        const uint32_t mask = (1u << width) - 1;
+        if (shift == beginBit) {
+            radixUpsweepKernel<K, ArraySource<K>><<<tiles, kSortThreads, 0, stream>>>(source, count, shift, mask, tiles, b.hist);
+        } else {
+            radixDigitUpsweepKernel<<<tiles, kSortThreads, 0, stream>>>(b.digits, count, tiles, b.hist);
+        }
+        exclusiveSum(stream, b.hist, b.hist, tiles * kRadixBins, b.aggregates);
+        if (nextShift < endBit) {
+            const uint32_t nextMask = (1u << std::min(kRadixBits, endBit - nextShift)) - 1;
+            radixDownsweepKernel<K, ArraySource<K>, true><<<tiles, kSortThreads, 0, stream>>>(
+                source, count, shift, mask, tiles, b.hist, b.keys[1 - b.current], b.values[1 - b.current], count, b.digits, nextShift, nextMask);
+        } else {
+            radixDownsweepKernel<K, ArraySource<K>><<<tiles, kSortThreads, 0, stream>>>(
+                source, count, shift, mask, tiles, b.hist, b.keys[1 - b.current], b.values[1 - b.current], count);
+        }
+        b.current = 1 - b.current;
     }
 }
 
