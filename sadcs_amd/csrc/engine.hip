@@ -104,11 +104,19 @@ __global__ void fillBytesKernel(uint8_t* __restrict__ out, uint64_t count, uint8
 
 // The bytes that occur in the texts, as a 256-bit mask (bit b mod 32 of word b / 32): the dense
 // codes need only which bytes occur, not how often. Each thread reads 16-byte chunks (the text
-// buffer is 16-byte aligned and padded) and keeps the mask in registers; gap bytes between the
-// texts of a batch are masked off (a chunk never spans a text's start, which is 256-aligned).
+// buffer is 16-byte aligned and padded); gap bytes between the texts of a batch are masked off (a
+// chunk never spans a text's start, which is 256-aligned). Every thread ORs each byte's bit into
+// its own 8 words of LDS, one LDS atomic per byte with no dependent read; 9 words apart, the
+// lanes' words fall in distinct banks. (Selecting the word in registers cost 8 compares and
+// selects per byte: the kernel was VALU-bound at a third of the text's read rate.)
 template <class Texts>
 __global__ __launch_bounds__(256) void bytePresenceKernel(const uint4* __restrict__ chunks, uint32_t span, Texts texts, uint32_t* __restrict__ present) {
-    uint32_t mask[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    constexpr uint32_t kStride = 9;
+    __shared__ uint32_t bits[256 * kStride];
+    uint32_t* mine = bits + threadIdx.x * kStride;
+    for (uint32_t x = 0; x < 8; x++) {
+        mine[x] = 0;
+    }
     const uint32_t count = static_cast<uint32_t>((static_cast<uint64_t>(span) + 15) / 16);
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < count; c += gridDim.x * blockDim.x) {
         const uint32_t p0 = c * 16;
@@ -119,26 +127,24 @@ __global__ __launch_bounds__(256) void bytePresenceKernel(const uint4* __restric
 #pragma unroll
         for (uint32_t k = 0; k < 16; k++) {
             const uint32_t b = (words[k >> 2] >> ((k & 3) * 8)) & 255u;
-            const uint32_t bit = k < valid ? 1u << (b & 31) : 0u;
-#pragma unroll
-            for (uint32_t x = 0; x < 8; x++) {
-                mask[x] |= (b >> 5) == x ? bit : 0u;
+            if (k < valid) {
+                atomicOr(&mine[b >> 5], 1u << (b & 31));
             }
         }
     }
-#pragma unroll
-    for (uint32_t x = 0; x < 8; x++) {
-        for (int offset = kWave / 2; offset > 0; offset /= 2) {
-            mask[x] |= __shfl_xor(mask[x], offset);
-        }
+    __syncthreads();
+    // Thread t ORs word t % 8 of every 32nd thread from t / 8, then the wave's lanes with the same
+    // word (lane bits 3-5) combine by shuffles; lanes 0-7 publish the wave's words.
+    const uint32_t x = threadIdx.x & 7;
+    uint32_t mask = 0;
+    for (uint32_t l = threadIdx.x >> 3; l < 256; l += 32) {
+        mask |= bits[l * kStride + x];
     }
-    if (threadIdx.x % kWave == 0) {
-#pragma unroll
-        for (uint32_t x = 0; x < 8; x++) {
-            if (mask[x]) {
-                atomicOr(&present[x], mask[x]);
-            }
-        }
+    for (int offset = 8; offset < kWave; offset *= 2) {
+        mask |= __shfl_xor(mask, offset);
+    }
+    if (threadIdx.x % kWave < 8 && mask) {
+        atomicOr(&present[x], mask);
     }
 }
 
