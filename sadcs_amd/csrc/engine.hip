@@ -795,12 +795,17 @@ class Pipeline {
             }
 
             const int y = 1 - listBuffer;
+            // The first pass of a single text sorts the identity as its values (t = index) without
+            // storing them, and its key kernel writes the first digit for the radix histograms.
+            const bool identity = first && !kBatch && u > 1;
+            uint32_t* valuesOut = identity ? nullptr : values[y];
+            uint8_t* digitsOut = first ? digits : nullptr;
             if (sigma == 256 && !spec.doubling) {
                 makePassKeysKernel<V, Texts, true><<<blocksFor(u), 256, 0, m_stream>>>(
-                    rp, m_texts, spec, codes, first, values[listBuffer], reinterpret_cast<const uint32_t*>(keys[listBuffer]), u, keys[y], values[y]);
+                    rp, m_texts, spec, codes, first, values[listBuffer], reinterpret_cast<const uint32_t*>(keys[listBuffer]), u, keys[y], valuesOut, digitsOut);
             } else {
                 makePassKeysKernel<V, Texts, false><<<blocksFor(u), 256, 0, m_stream>>>(
-                    rp, m_texts, spec, codes, first, values[listBuffer], reinterpret_cast<const uint32_t*>(keys[listBuffer]), u, keys[y], values[y]);
+                    rp, m_texts, spec, codes, first, values[listBuffer], reinterpret_cast<const uint32_t*>(keys[listBuffer]), u, keys[y], valuesOut, digitsOut);
             }
             // After the first pass the unresolved groups are mostly pairs: sort within groups, and
             // fall back to the radix sort for a pass with a group too large for that.
@@ -817,7 +822,11 @@ class Pipeline {
             }
             if (!sorted) {
                 sb.current = y;
+                sb.identityValues = identity;
+                sb.digitsReady = first;
                 radixSort<uint64_t>(m_stream, sb, u, 0, ordinalBits + spec.subkeyBits);
+                sb.identityValues = false;
+                sb.digitsReady = false;
             }
             const int c = sb.current;
 

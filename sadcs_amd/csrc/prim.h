@@ -274,11 +274,11 @@ constexpr int kSortWaves = kSortThreads / kWave;
 template <class K>
 struct ArraySource {
     const K* __restrict__ keys;
-    const uint32_t* __restrict__ values;
+    const uint32_t* __restrict__ values;  // null: value i (the identity)
 
     __device__ __forceinline__ bool operator()(uint32_t i, K& key, uint32_t& value) const {
         key = keys[i];
-        value = values[i];
+        value = values ? values[i] : i;
         return true;
     }
 };
@@ -336,10 +336,12 @@ __global__ __launch_bounds__(kSortThreads) void radixUpsweepKernel(Source source
 // items' order (radixDownsweepKernel with kDigits): a tile reads 1 byte per item instead of its
 // keys. hist as radixUpsweepKernel writes it.
 __global__ __launch_bounds__(kSortThreads) void radixDigitUpsweepKernel(const uint8_t* __restrict__ digits, uint32_t count, uint32_t tiles, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t counts[kSortWaves][kRadixBins];
-    const uint32_t wave = threadIdx.x / kWave;
+    // Two histograms per wave (even and odd lanes): a skewed digit's lanes collide half as often.
+    constexpr int kCopies = 2 * kSortWaves;
+    __shared__ uint32_t counts[kCopies][kRadixBins];
+    const uint32_t copy = 2 * (threadIdx.x / kWave) + (threadIdx.x & 1);
 
-    for (uint32_t x = threadIdx.x; x < kSortWaves * kRadixBins; x += kSortThreads) {
+    for (uint32_t x = threadIdx.x; x < kCopies * kRadixBins; x += kSortThreads) {
         (&counts[0][0])[x] = 0;
     }
     __syncthreads();
@@ -362,14 +364,14 @@ __global__ __launch_bounds__(kSortThreads) void radixDigitUpsweepKernel(const ui
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
         if (first + j < count) {
-            atomicAdd(&counts[wave][static_cast<uint32_t>(packed >> (8 * j)) & 0xffu], 1u);
+            atomicAdd(&counts[copy][static_cast<uint32_t>(packed >> (8 * j)) & 0xffu], 1u);
         }
     }
     __syncthreads();
 
     for (uint32_t d = threadIdx.x; d < kRadixBins; d += kSortThreads) {
         uint32_t total = 0;
-        for (int w = 0; w < kSortWaves; w++) {
+        for (int w = 0; w < kCopies; w++) {
             total += counts[w][d];
         }
         hist[static_cast<uint64_t>(d) * tiles + tile] = total;
@@ -493,6 +495,8 @@ struct SortBuffers {
     uint32_t* hist = nullptr;
     Sum1* aggregates = nullptr;
     uint8_t* digits = nullptr;  // optional, one byte per item: radixSort's passes hand on their next digit
+    bool digitsReady = false;   // digits already holds the first pass's digit (written with the keys)
+    bool identityValues = false;  // the values are 0, 1, 2, ... and not stored: the first pass makes them
 };
 
 inline size_t radixHistWords(uint64_t capacity) {
@@ -525,16 +529,18 @@ inline void radixSort(hipStream_t stream, SortBuffers<K>& b, uint32_t count, int
     const uint32_t tiles = std::max<uint32_t>(1, ceilDiv(count, kSortTile));
     for (int shift = beginBit; shift < endBit; shift += kRadixBits) {
         const int width = std::min(kRadixBits, endBit - shift);
-        const ArraySource<K> source{b.keys[b.current], b.values[b.current]};
+        const bool firstPass = shift == beginBit;
+        const ArraySource<K> source{b.keys[b.current], firstPass && b.identityValues ? nullptr : b.values[b.current]};
         const int nextShift = shift + kRadixBits;
-        if (!b.digits || (shift == beginBit && nextShift >= endBit)) {
+        if (!b.digits || (firstPass && nextShift >= endBit && !b.digitsReady)) {
             radixPass<K>(stream, b, source, count, shift, width);
             continue;
         }
-        // With a digit buffer: every pass after the first counts the bytes the one before wrote.
-        // The buffer is read by this pass's histograms before its scatter overwrites it.
+        // With a digit buffer: every pass after the first counts the bytes the one before wrote
+        // (the first too when the keys' writer wrote its digits). The buffer is read by this pass's
+        // histograms before its scatter overwrites it.
         const uint32_t mask = (1u << width) - 1;
-        if (shift == beginBit) {
+        if (firstPass && !b.digitsReady) {
             radixUpsweepKernel<K, ArraySource<K>><<<tiles, kSortThreads, 0, stream>>>(source, count, shift, mask, tiles, b.hist);
         } else {
             radixDigitUpsweepKernel<<<tiles, kSortThreads, 0, stream>>>(b.digits, count, tiles, b.hist);

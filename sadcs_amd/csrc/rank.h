@@ -86,10 +86,12 @@ __device__ __forceinline__ void firstPassEntry(const RankParams& rp, uint32_t i,
     }
 }
 
-// Keys of one pass: key = (ordinal << subkeyBits) | subkey, value = compact index t.
+// Keys of one pass: key = (ordinal << subkeyBits) | subkey, value = compact index t (not written
+// when valuesOut is null: the first pass of a single text, whose t is the index). digitsOut, if
+// not null, gets the key's low byte, the first digit of the radix sort.
 // kWideCodes: all 256 byte values occur, the code is the byte plus one (9 bits).
 template <int V, class Texts, bool kWideCodes>
-__global__ __launch_bounds__(256) void makePassKeysKernel(RankParams rp, Texts texts, PassSpec spec, CodeTable codes, bool first, const uint32_t* __restrict__ listT, const uint32_t* __restrict__ listOrdinal, uint32_t u, uint64_t* __restrict__ keysOut, uint32_t* __restrict__ valuesOut) {
+__global__ __launch_bounds__(256) void makePassKeysKernel(RankParams rp, Texts texts, PassSpec spec, CodeTable codes, bool first, const uint32_t* __restrict__ listT, const uint32_t* __restrict__ listOrdinal, uint32_t u, uint64_t* __restrict__ keysOut, uint32_t* __restrict__ valuesOut, uint8_t* __restrict__ digitsOut) {
     __shared__ uint8_t code[256];
     for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) {
         code[x] = codes.code[x];
@@ -131,8 +133,14 @@ __global__ __launch_bounds__(256) void makePassKeysKernel(RankParams rp, Texts t
         }
     }
 
-    keysOut[i] = (spec.subkeyBits < 64 ? (uint64_t(ordinal) << spec.subkeyBits) : 0) | subkey;
-    valuesOut[i] = t;
+    const uint64_t key = (spec.subkeyBits < 64 ? (uint64_t(ordinal) << spec.subkeyBits) : 0) | subkey;
+    keysOut[i] = key;
+    if (valuesOut) {
+        valuesOut[i] = t;
+    }
+    if (digitsOut) {
+        digitsOut[i] = static_cast<uint8_t>(key);
+    }
 }
 
 // The sort of a pass after the first, when every group is small. The list arrives grouped: keys
