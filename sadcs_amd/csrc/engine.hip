@@ -264,6 +264,17 @@ class Arena {
 
     template <class T>
     int allocate(T*& pointer, size_t count) {
+        if (tryAllocate(pointer, count)) {
+            return SADCS_OK;
+        }
+        const size_t bytes = std::max<size_t>(kAlign, (count * sizeof(T) + kAlign - 1) / kAlign * kAlign);
+        setError("device arena exhausted: %zu bytes requested, capacity %zu", bytes, m_capacity);
+        return SADCS_NOT_ENOUGH_MEMORY;
+    }
+
+    // allocate() without an error when there is no room: for optional buffers.
+    template <class T>
+    bool tryAllocate(T*& pointer, size_t count) {
         const size_t bytes = std::max<size_t>(kAlign, (count * sizeof(T) + kAlign - 1) / kAlign * kAlign);
         for (size_t i = 0; i < m_free.size(); i++) {
             if (m_free[i].size >= bytes) {
@@ -277,11 +288,10 @@ class Arena {
                 m_highWater = std::max(m_highWater, offset + bytes);
                 m_live += bytes;
                 pointer = reinterpret_cast<T*>((m_dry ? reinterpret_cast<uint8_t*>(kAlign) : m_base) + offset);
-                return SADCS_OK;
+                return true;
             }
         }
-        setError("device arena exhausted: %zu bytes requested, capacity %zu", bytes, m_capacity);
-        return SADCS_NOT_ENOUGH_MEMORY;
+        return false;
     }
 
     template <class T>
@@ -527,6 +537,11 @@ uint64_t classSizeOf(uint64_t n, int d) {
     return size;
 }
 
+// A pass after the first sorts within groups only while its groups hold at most this many
+// elements on average (u / groups); larger groups make the per-element scans cost more than the
+// radix passes.
+constexpr uint64_t kGroupRankMeanMax = 8;
+
 // The first ranking pass of a large sample leaves its resolved rank stores to
 // scatterResolvedRanksKernel, in this many launches over consecutive rank ranges.
 constexpr uint32_t kDeferredRankMin = 1u << 22;
@@ -666,6 +681,45 @@ class Pipeline {
     }
 
     // ---- phase (a) ------------------------------------------------------------------------------
+
+    // The elements of large groups (flags from groupRankSortKernel, `large` of them): compacted,
+    // sorted by radix and placed at their list positions in keys/values[1 - y]. sorted = false
+    // (and nothing done) when the arena has no room for them.
+    int sortLargeGroups(uint64_t* const* keys, uint32_t* const* values, int y, uint32_t u, uint32_t large, int keyBits,
+                        uint8_t* flags, uint32_t* hist, Sum1* sortAggregates, Sum1* scanAggregates, bool& sorted) {
+        uint64_t* subKeys[2] = {};
+        uint32_t* subValues[2] = {};
+        uint32_t* subPos = nullptr;
+        bool room = true;
+        for (int b = 0; b < 2 && room; b++) {
+            room = m_arena.tryAllocate(subKeys[b], large) && m_arena.tryAllocate(subValues[b], large);
+        }
+        room = room && m_arena.tryAllocate(subPos, large);
+        if (room) {
+            scanPipeline<Sum1, kSumItems>(m_stream, u, LargeFlagSource{flags},
+                                          LargeGroupCompact{flags, keys[y], values[y], subKeys[0], subValues[0], subPos}, scanAggregates);
+            SortBuffers<uint64_t> sub;
+            for (int b = 0; b < 2; b++) {
+                sub.keys[b] = subKeys[b];
+                sub.values[b] = subValues[b];
+            }
+            sub.hist = hist;
+            sub.aggregates = sortAggregates;
+            sub.digits = flags;  // read by the compaction above, before the sort writes digits
+            radixSort<uint64_t>(m_stream, sub, large, 0, keyBits);
+            placeLargeGroupsKernel<<<blocksFor(large), 256, 0, m_stream>>>(sub.keys[sub.current], sub.values[sub.current], subPos, large,
+                                                                          keys[1 - y], values[1 - y]);
+            SADCS_HIP(hipGetLastError());
+            sorted = true;
+        }
+        // Stream order keeps these kernels' uses ahead of any reuse of the memory.
+        m_arena.release(subPos);
+        for (int b = 1; b >= 0; b--) {
+            m_arena.release(subValues[b]);
+            m_arena.release(subKeys[b]);
+        }
+        return SADCS_OK;
+    }
 
     int rankSample() {
         const uint64_t rankWords = static_cast<uint64_t>(Cover::pack(m_job.span)) + CoverSet<V>::kSize + 8;
@@ -807,17 +861,25 @@ class Pipeline {
                 makePassKeysKernel<V, Texts, false><<<blocksFor(u), 256, 0, m_stream>>>(
                     rp, m_texts, spec, codes, first, values[listBuffer], reinterpret_cast<const uint32_t*>(keys[listBuffer]), u, keys[y], valuesOut, digitsOut);
             }
-            // After the first pass the unresolved groups are mostly pairs: sort within groups, and
-            // fall back to the radix sort for a pass with a group too large for that.
+            // After the first pass the unresolved groups are mostly pairs: sort within groups. The
+            // few elements of groups too large for that (long runs) are sorted apart; when they are
+            // many, or the groups are large on average (the scans would cost more than the radix
+            // passes), the pass is sorted by radix.
             bool sorted = false;
-            if (!first && ordinalBits > 0 && spec.subkeyBits < 64) {
+            if (!first && ordinalBits > 0 && spec.subkeyBits < 64 && u <= kGroupRankMeanMax * static_cast<uint64_t>(groups)) {
                 SADCS_HIP(hipMemsetAsync(counts + 2, 0, sizeof(uint32_t), m_stream));
-                groupRankSortKernel<<<blocksFor(u), 256, 0, m_stream>>>(keys[y], values[y], u, spec.subkeyBits, keys[1 - y], values[1 - y], counts + 2);
+                groupRankSortKernel<<<blocksFor(u), 256, 0, m_stream>>>(keys[y], values[y], u, spec.subkeyBits, keys[1 - y], values[1 - y], digits, counts + 2);
                 SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords + 2, counts + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
                 SADCS_HIP(hipStreamSynchronize(m_stream));
-                if (m_ctx.hostWords[2] == 0) {
-                    sb.current = 1 - y;
+                const uint32_t large = m_ctx.hostWords[2];
+                if (large == 0) {
                     sorted = true;
+                } else if (large <= u / 4) {
+                    SADCS_TRY(sortLargeGroups(keys, values, y, u, large, ordinalBits + spec.subkeyBits, digits, hist, sortAggregates,
+                                              reinterpret_cast<Sum1*>(markAggregates), sorted));
+                }
+                if (sorted) {
+                    sb.current = 1 - y;
                 }
             }
             if (!sorted) {

@@ -143,51 +143,99 @@ __global__ __launch_bounds__(256) void makePassKeysKernel(RankParams rp, Texts t
     }
 }
 
-// The sort of a pass after the first, when every group is small. The list arrives grouped: keys
+// The sort of a pass after the first, when its groups are small. The list arrives grouped: keys
 // are (ordinal << subkeyBits) | subkey with ordinals non-decreasing along the list (the previous
 // pass compacted its unresolved groups in sorted order). The stable sort of such a list puts
 // element i at its group's start plus the members with a smaller key, plus those with an equal key
 // before i: the same order as the stable radix sort, with each element reading only its group.
 // Unresolved groups are mostly pairs (a repeat and its source), so this replaces the several
-// digit passes of a 50- to 64-bit radix sort. A group of more than kGroupRankMax elements sets
-// *large and the caller sorts the pass by radix instead (the output is then ignored). Ordinals
-// do not decrease along the list, so one read kGroupRankMax away shows whether the group is
-// larger than that: elements of a huge group (periodic text) give up at once instead of scanning.
+// digit passes of a 50- to 64-bit radix sort. An element of a group of more than kGroupRankMax
+// elements is not placed: it sets largeFlags[i] (0 otherwise) and counts itself in *largeCount,
+// and the caller sorts those elements apart (or the whole pass by radix). Ordinals do not decrease
+// along the list, so one read kGroupRankMax away shows whether the group is larger than that:
+// elements of a huge group (a long run) stop at once instead of scanning.
 constexpr uint32_t kGroupRankMax = 256;
 
-__global__ void groupRankSortKernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ values, uint32_t u, int subkeyBits, uint64_t* __restrict__ keysOut, uint32_t* __restrict__ valuesOut, uint32_t* __restrict__ large) {
+__global__ __launch_bounds__(256) void groupRankSortKernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ values, uint32_t u, int subkeyBits, uint64_t* __restrict__ keysOut, uint32_t* __restrict__ valuesOut, uint8_t* __restrict__ largeFlags, uint32_t* __restrict__ largeCount) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= u) {
-        return;
+    bool large = false;
+    if (i < u) {
+        const uint64_t key = keys[i];
+        const uint64_t ordinal = key >> subkeyBits;
+        large = (i >= kGroupRankMax && (keys[i - kGroupRankMax] >> subkeyBits) == ordinal) ||
+                (i + kGroupRankMax < u && (keys[i + kGroupRankMax] >> subkeyBits) == ordinal);
+        if (!large) {
+            uint32_t start = i;
+            for (uint32_t step = 0; step < kGroupRankMax && start > 0 && (keys[start - 1] >> subkeyBits) == ordinal; step++) {
+                start--;
+            }
+            uint32_t end = i + 1;
+            for (uint32_t step = 0; step < kGroupRankMax && end < u && (keys[end] >> subkeyBits) == ordinal; step++) {
+                end++;
+            }
+            const bool leftOpen = start > 0 && (keys[start - 1] >> subkeyBits) == ordinal;
+            const bool rightOpen = end < u && (keys[end] >> subkeyBits) == ordinal;
+            large = leftOpen || rightOpen || end - start > kGroupRankMax;
+            if (!large) {
+                uint32_t rank = start;
+                for (uint32_t j = start; j < end; j++) {
+                    const uint64_t other = keys[j];
+                    rank += (other < key || (other == key && j < i)) ? 1u : 0u;
+                }
+                keysOut[rank] = key;
+                valuesOut[rank] = values[i];
+            }
+        }
+        largeFlags[i] = large ? 1 : 0;
     }
-    const uint64_t key = keys[i];
-    const uint64_t ordinal = key >> subkeyBits;
-    if ((i >= kGroupRankMax && (keys[i - kGroupRankMax] >> subkeyBits) == ordinal) ||
-        (i + kGroupRankMax < u && (keys[i + kGroupRankMax] >> subkeyBits) == ordinal)) {
-        atomicOr(large, 1u);
-        return;
+    const uint64_t wave = ballot(large);
+    if (wave != 0 && laneId() == static_cast<uint32_t>(__builtin_ctzll(wave))) {
+        atomicAdd(largeCount, static_cast<uint32_t>(__builtin_popcountll(wave)));
     }
-    uint32_t start = i;
-    for (uint32_t step = 0; step < kGroupRankMax && start > 0 && (keys[start - 1] >> subkeyBits) == ordinal; step++) {
-        start--;
+}
+
+// The elements of large groups, in list order: subKeys/subValues to be sorted, and subPos, the
+// list positions they came from (ascending). Sorted stably by their keys, the k-th of them belongs
+// at subPos[k]: the large groups keep their order (ordinals ascend along the list) and each
+// occupies a contiguous range of positions.
+struct LargeFlagSource {
+    const uint8_t* flags;
+    __device__ Vec<1, 0u> operator()(uint32_t i) const {
+        Vec<1, 0u> r;
+        r.v[0] = flags[i];
+        return r;
     }
-    uint32_t end = i + 1;
-    for (uint32_t step = 0; step < kGroupRankMax && end < u && (keys[end] >> subkeyBits) == ordinal; step++) {
-        end++;
+};
+
+struct LargeGroupCompact {
+    const uint8_t* flags;
+    const uint64_t* keys;
+    const uint32_t* values;
+    uint64_t* subKeys;
+    uint32_t* subValues;
+    uint32_t* subPos;
+    struct Prepared {};
+    __device__ Prepared prepare(uint32_t) const {
+        return {};
     }
-    const bool leftOpen = start > 0 && (keys[start - 1] >> subkeyBits) == ordinal;
-    const bool rightOpen = end < u && (keys[end] >> subkeyBits) == ordinal;
-    if (leftOpen || rightOpen || end - start > kGroupRankMax) {
-        atomicOr(large, 1u);
-        return;
+    __device__ void complete(Prepared&) const {}
+    __device__ void operator()(uint32_t i, const Prepared&, const Vec<1, 0u>& item, const Vec<1, 0u>& exclusive, bool valid) const {
+        if (valid && item.v[0] != 0) {
+            const uint32_t k = exclusive.v[0];
+            subKeys[k] = keys[i];
+            subValues[k] = values[i];
+            subPos[k] = i;
+        }
     }
-    uint32_t rank = start;
-    for (uint32_t j = start; j < end; j++) {
-        const uint64_t other = keys[j];
-        rank += (other < key || (other == key && j < i)) ? 1u : 0u;
+};
+
+__global__ void placeLargeGroupsKernel(const uint64_t* __restrict__ subKeys, const uint32_t* __restrict__ subValues, const uint32_t* __restrict__ subPos, uint32_t count, uint64_t* __restrict__ keysOut, uint32_t* __restrict__ valuesOut) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) {
+        const uint32_t at = subPos[k];
+        keysOut[at] = subKeys[k];
+        valuesOut[at] = subValues[k];
     }
-    keysOut[rank] = key;
-    valuesOut[rank] = values[i];
 }
 
 // Four fields: last head (max), group start (max), unresolved before (sum), unresolved groups before (sum).

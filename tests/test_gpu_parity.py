@@ -100,6 +100,40 @@ def test_random_alphabets(gpu):
             check_case(gpu, f"random sigma={sigma} #{i}", rng.random_text(n, sigma))
 
 
+def runs_among_repeats(n, seed):
+    """Random ACGT text with planted repeats (many groups of two after the first pass) and a few
+    long runs (one group of thousands of samples): passes after the first place the small groups
+    in place and sort the run's group apart."""
+    rng = np.random.default_rng(seed)
+    text = rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), n)
+    for _ in range(n // 400):
+        length = int(rng.integers(200, 600))
+        src, dst = (int(x) for x in rng.integers(0, n - length, 2))
+        text[dst:dst + length] = text[src:src + length].copy()
+    for length in (5000, 3000, 2000, 1500):
+        at = int(rng.integers(0, n - length))
+        text[at:at + length] = ord("N")
+    at = int(rng.integers(0, n - 3000))
+    text[at:at + 3000] = np.frombuffer(b"ab" * 1500, dtype=np.uint8)
+    return text.tobytes()
+
+
+def test_runs_among_repeats(gpu):
+    for seed, n in [(1, 300_000), (2, 1 << 20)]:
+        text = runs_among_repeats(n, seed)
+        check_case(gpu, f"runs among repeats seed={seed}", text)
+        check_ranks(gpu, f"runs among repeats seed={seed}", text)
+
+
+def test_batch_runs_among_repeats(gpu):
+    texts = [runs_among_repeats(n, 10 + i) for i, n in enumerate([200_000, 500_000, 350_000])]
+    texts.append(oracle.Rng(5).random_text(100_000, 200))
+    for i, (sa, bwt, primary) in enumerate(gpu.gpu_batch_build(texts)):
+        want = expected_sa(texts[i])
+        assert np.array_equal(sa, want), i
+        assert (bwt, primary) == oracle.bwt_from_sa(texts[i], want), i
+
+
 @pytest.mark.skipif(not oracle.ref_available(), reason="needs the reference build in oracle/_ref")
 def test_large(gpu):
     rng = oracle.Rng(3)
