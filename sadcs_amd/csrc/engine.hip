@@ -742,9 +742,15 @@ class Pipeline {
         uint32_t* counts = nullptr;
         uint32_t* byteCounts = nullptr;
         uint8_t* digits = nullptr;  // the radix passes' next digits, one byte per sample
+        uint32_t* textMasks = nullptr;  // a batch's per-text alphabets, for its first pass
+        uint8_t* textCodes = nullptr;
+        uint8_t* textShape = nullptr;
 
         int status = SADCS_OK;
         auto cleanup = [&]() {
+            m_arena.release(textShape);
+            m_arena.release(textCodes);
+            m_arena.release(textMasks);
             unscratch(digits);
             m_arena.release(byteCounts);
             m_arena.release(counts);
@@ -769,6 +775,11 @@ class Pipeline {
         if (status == SADCS_OK) status = m_arena.allocate(counts, 4);
         if (status == SADCS_OK) status = m_arena.allocate(byteCounts, 8);
         if (status == SADCS_OK) status = scratch(digits, std::max<uint32_t>(M, 1));
+        if (kBatch && m_segments > 1) {
+            if (status == SADCS_OK) status = m_arena.allocate(textMasks, static_cast<uint64_t>(m_segments) * 8);
+            if (status == SADCS_OK) status = m_arena.allocate(textCodes, static_cast<uint64_t>(m_segments) * 256);
+            if (status == SADCS_OK) status = m_arena.allocate(textShape, static_cast<uint64_t>(m_segments) * 2);
+        }
         if (status != SADCS_OK || m_dry || M == 0) {
             cleanup();
             return status;
@@ -805,6 +816,20 @@ class Pipeline {
         rp.samples = M;
         rp.segments = m_segments;
         rp.segmentTable = m_job.deviceSegments;
+        // A batch whose texts have different alphabets: the first pass packs each text's own
+        // dense codes, as many characters as fit (the later passes use the common codes).
+        const int firstBits = std::min(56, 64 - (m_segments > 1 ? bitWidth(m_segments - 1) : 0));
+        if constexpr (kBatch) {
+            if (textCodes && sigma < 256) {
+                fillWords(textMasks, m_segments * 8, 0);
+                textPresenceKernel<<<static_cast<unsigned>((static_cast<uint64_t>(m_job.span) + 65535) / 65536), 256, 0, m_stream>>>(
+                    reinterpret_cast<const uint4*>(m_job.text), m_job.span, m_texts, textMasks);
+                textCodesKernel<<<blocksFor(m_segments), 256, 0, m_stream>>>(textMasks, m_segments, firstBits, textCodes, textShape);
+                SADCS_HIP(hipGetLastError());
+                rp.textCodes = textCodes;
+                rp.textShape = textShape;
+            }
+        }
 
         initGroupIdsKernel<<<blocksFor(m_segments), 256, 0, m_stream>>>(m_job.deviceSegments, m_segments, groupIds[0]);
 
@@ -836,6 +861,11 @@ class Pipeline {
                 // (8) resolves nearly as many, and the later passes sort only the few left.
                 spec.chars = std::max(1, (first ? std::min(56, 64 - ordinalBits) : 64 - ordinalBits) / spec.charBits);
                 spec.subkeyBits = spec.chars * spec.charBits;
+                if (first && rp.textCodes) {
+                    // Per-text codes fill the first pass's bits; the depth still advances by the
+                    // common codes' characters, which every text's key covers.
+                    spec.subkeyBits = firstBits;
+                }
             } else {
                 const uint64_t h = depth / V * V;
                 if (h >= m_job.maxLength) {

@@ -47,6 +47,10 @@ struct RankParams {
     uint32_t samples;        // valid sample suffixes (all texts)
     uint32_t segments;
     const Segment* segmentTable;
+    // A batch's first pass: per text, dense codes of its own bytes (256 per text) and its code
+    // width and characters per key (2 per text); null otherwise.
+    const uint8_t* textCodes;
+    const uint8_t* textShape;
 };
 
 // Subkey of one pass: characters at depth `depth`, or the group id at p + h plus one.
@@ -117,6 +121,24 @@ __global__ __launch_bounds__(256) void makePassKeysKernel(RankParams rp, Texts t
     if (spec.doubling) {
         const uint32_t q = p + spec.offset;
         subkey = q < end ? uint64_t(rp.ranks[packPosition<V>(q)]) + 1 : 0;
+    } else if (first && rp.textCodes) {
+        // Texts of a batch never compare, so each packs its own codes as densely as they allow
+        // (a DNA text 3 bits per character among texts of 96 symbols), left-aligned in the
+        // subkey. The ordinal of a first-pass element is its text.
+        const uint8_t* table = rp.textCodes + static_cast<size_t>(ordinal) * 256;
+        const int bits = rp.textShape[2 * ordinal], chars = rp.textShape[2 * ordinal + 1];
+        const uint32_t q = p + spec.offset;
+        const uint32_t remaining = q < end ? end - q : 0;
+        uint64_t w = 0;
+        for (int c = 0; c < chars; c++) {
+            if ((c & 7) == 0) {
+                w = static_cast<uint32_t>(c) < remaining ? load8(rp.text, q + c) : 0;
+            }
+            const uint32_t byte = static_cast<uint32_t>(w >> 56);
+            w <<= 8;
+            subkey = (subkey << bits) | (static_cast<uint32_t>(c) < remaining ? table[byte] : 0u);
+        }
+        subkey <<= spec.subkeyBits - chars * bits;
     } else {
         const uint32_t q = p + spec.offset;
         const uint32_t remaining = q < end ? end - q : 0;
@@ -141,6 +163,79 @@ __global__ __launch_bounds__(256) void makePassKeysKernel(RankParams rp, Texts t
     if (digitsOut) {
         digitsOut[i] = static_cast<uint8_t>(key);
     }
+}
+
+// Per text of a batch: the bytes that occur in it (8 words per text, zeroed by the caller), for
+// codes dense within each text. Thread t of a workgroup takes 256-byte block t of the workgroup's
+// 64 KiB (a text starts on a block boundary, so the block is one text's or a gap's) into its own
+// LDS bitmap; the first thread of each run of threads on one text combines the run's bitmaps and
+// publishes them.
+__global__ __launch_bounds__(256) void textPresenceKernel(const uint4* __restrict__ chunks, uint32_t span, TextBatch texts, uint32_t* __restrict__ textMasks) {
+    constexpr uint32_t kStride = 9;
+    __shared__ uint32_t bits[256 * kStride];
+    __shared__ uint32_t runText[256];
+    uint32_t* mine = bits + threadIdx.x * kStride;
+    for (uint32_t x = 0; x < 8; x++) {
+        mine[x] = 0;
+    }
+    const uint64_t p0 = (static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x) << kSegmentBlockLog;
+    uint32_t text = ~0u;
+    if (p0 < span) {
+        const uint32_t segment = texts.segmentOf(static_cast<uint32_t>(p0));
+        const Segment s = texts.segments[segment];
+        const uint32_t end = min(s.end, span);
+        if (p0 >= s.start && p0 < end) {
+            text = segment;
+            const uint32_t valid = min(1u << kSegmentBlockLog, end - static_cast<uint32_t>(p0));
+            for (uint32_t k = 0; k * 16 < valid; k++) {
+                const uint4 q = chunks[(p0 >> 4) + k];
+                const uint32_t words[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                for (uint32_t j = 0; j < 16; j++) {
+                    const uint32_t b = (words[j >> 2] >> ((j & 3) * 8)) & 255u;
+                    if (k * 16 + j < valid) {
+                        atomicOr(&mine[b >> 5], 1u << (b & 31));
+                    }
+                }
+            }
+        }
+    }
+    runText[threadIdx.x] = text;
+    __syncthreads();
+    if (text != ~0u && (threadIdx.x == 0 || runText[threadIdx.x - 1] != text)) {
+        uint32_t m[8];
+        for (uint32_t x = 0; x < 8; x++) {
+            m[x] = mine[x];
+        }
+        for (uint32_t t = threadIdx.x + 1; t < 256 && runText[t] == text; t++) {
+            for (uint32_t x = 0; x < 8; x++) {
+                m[x] |= bits[t * kStride + x];
+            }
+        }
+        for (uint32_t x = 0; x < 8; x++) {
+            if (m[x]) {
+                atomicOr(&textMasks[static_cast<size_t>(text) * 8 + x], m[x]);
+            }
+        }
+    }
+}
+
+// Per text: codes 1..sigma of its bytes in byte order (0 for absent bytes; sigma < 256), the code
+// width and the characters that fit subkeyBits.
+__global__ void textCodesKernel(const uint32_t* __restrict__ textMasks, uint32_t texts, int subkeyBits, uint8_t* __restrict__ textCodes, uint8_t* __restrict__ textShape) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= texts) {
+        return;
+    }
+    uint32_t count = 0;
+    for (uint32_t b = 0; b < 256; b++) {
+        const uint32_t present = (textMasks[static_cast<size_t>(s) * 8 + b / 32] >> (b % 32)) & 1u;
+        count += present;
+        textCodes[static_cast<size_t>(s) * 256 + b] = static_cast<uint8_t>(present ? count : 0);
+    }
+    const int bits = count > 1 ? 32 - __clz(count) : 1;
+    textShape[2 * s] = static_cast<uint8_t>(bits);
+    textShape[2 * s + 1] = static_cast<uint8_t>(max(1, subkeyBits / bits));
 }
 
 // The sort of a pass after the first, when its groups are small. The list arrives grouped: keys
