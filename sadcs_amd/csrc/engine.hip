@@ -989,6 +989,7 @@ class Pipeline {
         Sum1* aggregates = nullptr;
         uint32_t* walkKeys = nullptr;  // the walks of classes that read the text, evaluated once
         uint32_t* walkValues = nullptr;
+        uint8_t* classDigits = nullptr;  // a batch's digit bytes between its passes over the text index
         const uint64_t walk = largest + m_segments + static_cast<uint64_t>(m_listLength[0]);
         uint64_t textWalk = 0;
         for (int d = 1; d < kClasses; d++) {
@@ -998,6 +999,7 @@ class Pipeline {
         }
         int status = SADCS_OK;
         auto cleanup = [&]() {
+            unscratch(classDigits);
             unscratch(walkValues);
             unscratch(walkKeys);
             unscratch(aggregates);
@@ -1017,6 +1019,7 @@ class Pipeline {
         if (status == SADCS_OK) status = scratch(aggregates, radixAggregateBytes(walk) / sizeof(Sum1) + 1);
         if (status == SADCS_OK) status = scratch(walkKeys, std::max<uint64_t>(textWalk, 1));
         if (status == SADCS_OK) status = scratch(walkValues, std::max<uint64_t>(textWalk, 1));
+        if (kBatch && status == SADCS_OK) status = scratch(classDigits, std::max<uint64_t>(largest, 1));
         if (status != SADCS_OK || m_dry) {
             cleanup();
             return status;
@@ -1053,16 +1056,24 @@ class Pipeline {
             sb.keys[1] = keys[next];
             sb.values[1] = m_lists[d];
             sb.current = passes % 2 == 1 ? 0 : 1;
+            // A batch's passes over the text index (bits 32 and up) count digit bytes that the pass
+            // before wrote, instead of reading the 8-byte keys.
+            const uint32_t kept = static_cast<uint32_t>(m_listLength[d]);
+            auto segmentWidth = [&](int e) { return std::min(kRadixBits, segmentBits - e * kRadixBits); };
+            sb.digits = classDigits;
+            const int firstNext = extraPasses > 0 && classDigits ? 32 : -1;
             if (fromText) {
                 materializeWalkKernel<decltype(source), Key><<<blocksFor(count), 256, 0, m_stream>>>(source, count, walkKeys, walkValues);
-                radixPass<Key>(m_stream, sb, MaterializedWalk<Key, Texts>{walkKeys, walkValues, m_texts}, count, 24, 8, static_cast<uint32_t>(m_listLength[d]));
+                radixPassDigits<Key>(m_stream, sb, MaterializedWalk<Key, Texts>{walkKeys, walkValues, m_texts}, count, 24, 8, kept,
+                                     false, firstNext, firstNext >= 0 ? segmentWidth(0) : 0);
             } else {
-                radixPass<Key>(m_stream, sb, source, count, 24, 8, static_cast<uint32_t>(m_listLength[d]));
+                radixPassDigits<Key>(m_stream, sb, source, count, 24, 8, kept, false, firstNext, firstNext >= 0 ? segmentWidth(0) : 0);
             }
             for (int e = 0; e < extraPasses; e++) {
                 const int shift = 32 + e * kRadixBits;
-                const int width = std::min(kRadixBits, 32 + segmentBits - shift);
-                radixPass<Key>(m_stream, sb, ArraySource<Key>{sb.keys[sb.current], sb.values[sb.current]}, static_cast<uint32_t>(m_listLength[d]), shift, width);
+                const int next = e + 1 < extraPasses && classDigits ? shift + kRadixBits : -1;
+                radixPassDigits<Key>(m_stream, sb, ArraySource<Key>{sb.keys[sb.current], sb.values[sb.current]}, kept, shift, segmentWidth(e),
+                                     0xFFFFFFFFu, classDigits != nullptr, next, next >= 0 ? segmentWidth(e + 1) : 0);
             }
             SADCS_HIP(hipGetLastError());
             prev = next;

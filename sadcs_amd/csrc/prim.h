@@ -520,6 +520,31 @@ inline void radixPass(hipStream_t stream, SortBuffers<K>& b, Source source, uint
     b.current = 1 - b.current;
 }
 
+// radixPass with digit bytes (b.digits): fromDigits, the histograms count the bytes the pass
+// before wrote instead of the source's keys; nextShift >= 0, the scatter writes each output
+// item's digit at nextShift (nextWidth bits) for the pass after.
+template <class K, class Source>
+inline void radixPassDigits(hipStream_t stream, SortBuffers<K>& b, Source source, uint32_t count, int shift, int width, uint32_t limit,
+                            bool fromDigits, int nextShift, int nextWidth) {
+    const uint32_t tiles = std::max<uint32_t>(1, ceilDiv(count, kSortTile));
+    const uint32_t mask = (1u << width) - 1;
+    if (fromDigits) {
+        radixDigitUpsweepKernel<<<tiles, kSortThreads, 0, stream>>>(b.digits, count, tiles, b.hist);
+    } else {
+        radixUpsweepKernel<K, Source><<<tiles, kSortThreads, 0, stream>>>(source, count, shift, mask, tiles, b.hist);
+    }
+    exclusiveSum(stream, b.hist, b.hist, tiles * kRadixBins, b.aggregates);
+    const uint32_t kept = limit == 0xFFFFFFFFu ? count : limit;
+    if (nextShift >= 0) {
+        radixDownsweepKernel<K, Source, true><<<tiles, kSortThreads, 0, stream>>>(
+            source, count, shift, mask, tiles, b.hist, b.keys[1 - b.current], b.values[1 - b.current], kept, b.digits, nextShift, (1u << nextWidth) - 1);
+    } else {
+        radixDownsweepKernel<K, Source><<<tiles, kSortThreads, 0, stream>>>(
+            source, count, shift, mask, tiles, b.hist, b.keys[1 - b.current], b.values[1 - b.current], kept);
+    }
+    b.current = 1 - b.current;
+}
+
 // Stable sort of b.keys[current], b.values[current] (count items) by bits [beginBit, endBit).
 template <class K>
 inline void radixSort(hipStream_t stream, SortBuffers<K>& b, uint32_t count, int beginBit, int endBit) {
