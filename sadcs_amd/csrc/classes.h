@@ -143,6 +143,121 @@ struct MaterializedWalk {
     }
 };
 
+// ---- a batch's class sort in one digit pass ---------------------------------------------------
+//
+// A batch sorts each class by (text, character) with key = text << 32 | characters. The walk is
+// every text's end first, then class d - 1 in (text, suffix) order. Over the list part alone the
+// items come in ascending text order, so one pass on the character digit can place them: a plain
+// pass puts an item at (kept list items of smaller digits) + (those of its digit before it), and
+// the second term exceeds its rank among its own text's items of its digit by exactly the items
+// of its digit in earlier texts. Its place in the class is its text's start plus its text's items
+// of smaller digits, plus its text's end if that has the same digit (the end comes first in the
+// walk), plus that rank. With per-text digit counts, the difference is one table lookup per
+// item (textDelta). The ends, one per text, go straight to the first place of their
+// (text, digit). Same order as the passes over the text index that it replaces.
+
+// The list part of a walk: walk index i + ends.
+template <class Source>
+struct ListPart {
+    Source source;
+    uint32_t ends;
+
+    template <class K>
+    __device__ __forceinline__ bool operator()(uint32_t i, K& key, uint32_t& value) const {
+        return source(i + ends, key, value);
+    }
+};
+
+// Per text s: its walk end (walk index s), kept or not; a kept end adds one to its text's count
+// of its digit (counts, text-major: the list items' counts already there).
+template <class Source, class K>
+__global__ void textEndsKernel(Source source, uint32_t texts, int shift, K* __restrict__ endKeys, uint32_t* __restrict__ endValues,
+                               uint32_t* __restrict__ endDigits, uint32_t* __restrict__ countsByText) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= texts) {
+        return;
+    }
+    K key = 0;
+    uint32_t value = 0;
+    uint32_t digit = ~0u;
+    if (source(s, key, value)) {
+        digit = static_cast<uint32_t>(key >> shift) & (kRadixBins - 1);
+        countsByText[static_cast<size_t>(s) * kRadixBins + digit] += 1;
+    }
+    endKeys[s] = key;
+    endValues[s] = value;
+    endDigits[s] = digit;
+}
+
+// byDigit[c * texts + s] = the list items of (s, c): all of them less the end.
+__global__ void listCountsByDigitKernel(const uint32_t* __restrict__ byText, const uint32_t* __restrict__ endDigits, uint32_t texts, uint32_t* __restrict__ byDigit) {
+    const uint64_t x = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (x < static_cast<uint64_t>(texts) * kRadixBins) {
+        const uint32_t s = static_cast<uint32_t>(x / kRadixBins), c = static_cast<uint32_t>(x % kRadixBins);
+        byDigit[static_cast<size_t>(c) * texts + s] = byText[x] - (endDigits[s] == c ? 1u : 0u);
+    }
+}
+
+// After the exclusive sums: byText[s][c] = first place of (s, c) in the class; byDigit[c][s] = a
+// plain pass's first place for the list items of (s, c). delta = the difference, past the end.
+__global__ void textDigitDeltaKernel(const uint32_t* __restrict__ byText, const uint32_t* __restrict__ byDigit, const uint32_t* __restrict__ endDigits,
+                                     uint32_t texts, uint32_t* __restrict__ delta) {
+    const uint64_t x = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (x < static_cast<uint64_t>(texts) * kRadixBins) {
+        const uint32_t s = static_cast<uint32_t>(x / kRadixBins), c = static_cast<uint32_t>(x % kRadixBins);
+        delta[x] = byText[x] + (endDigits[s] == c ? 1u : 0u) - byDigit[static_cast<size_t>(c) * texts + s];
+    }
+}
+
+template <class K>
+__global__ void placeTextEndsKernel(const K* __restrict__ endKeys, const uint32_t* __restrict__ endValues, const uint32_t* __restrict__ endDigits,
+                                    const uint32_t* __restrict__ byText, uint32_t texts, uint32_t limit, K* __restrict__ keysOut, uint32_t* __restrict__ valuesOut) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < texts && endDigits[s] != ~0u) {
+        const uint32_t at = byText[static_cast<size_t>(s) * kRadixBins + endDigits[s]];
+        if (at < limit) {
+            keysOut[at] = endKeys[s];
+            valuesOut[at] = endValues[s];
+        }
+    }
+}
+
+// Scratch of sortClassByText for `texts` texts.
+template <class K>
+struct TextSortScratch {
+    uint32_t* byText = nullptr;   // texts * 256
+    uint32_t* byDigit = nullptr;  // texts * 256
+    uint32_t* delta = nullptr;    // texts * 256
+    K* endKeys = nullptr;         // texts
+    uint32_t* endValues = nullptr;
+    uint32_t* endDigits = nullptr;
+};
+
+// One class of a batch: the walk (ends + list, `count` items, `kept` of them kept) sorted by
+// (text, digit at shift) into buffer 1 - b.current. b.aggregates scans texts * 256 words.
+template <class K, class Source>
+inline void sortClassByText(hipStream_t stream, SortBuffers<K>& b, Source source, uint32_t count, uint32_t kept, uint32_t texts, int shift,
+                            const TextSortScratch<K>& t) {
+    const uint32_t cells = texts * kRadixBins;
+    const uint32_t listCount = count - texts;
+    const uint32_t tiles = std::max<uint32_t>(1, ceilDiv(listCount, kSortTile));
+    const uint32_t mask = kRadixBins - 1;
+    const ListPart<Source> list{source, texts};
+    fillKernelWords(stream, t.byText, cells);
+    radixUpsweepKernel<K, ListPart<Source>, true><<<tiles, kSortThreads, 0, stream>>>(list, listCount, shift, mask, tiles, b.hist, t.byText);
+    exclusiveSum(stream, b.hist, b.hist, tiles * kRadixBins, b.aggregates);
+    textEndsKernel<Source, K><<<ceilDiv(texts, 256), 256, 0, stream>>>(source, texts, shift, t.endKeys, t.endValues, t.endDigits, t.byText);
+    listCountsByDigitKernel<<<ceilDiv(cells, 256), 256, 0, stream>>>(t.byText, t.endDigits, texts, t.byDigit);
+    exclusiveSum(stream, t.byText, t.byText, cells, b.aggregates);
+    exclusiveSum(stream, t.byDigit, t.byDigit, cells, b.aggregates);
+    textDigitDeltaKernel<<<ceilDiv(cells, 256), 256, 0, stream>>>(t.byText, t.byDigit, t.endDigits, texts, t.delta);
+    radixDownsweepKernel<K, ListPart<Source>><<<tiles, kSortThreads, 0, stream>>>(
+        list, listCount, shift, mask, tiles, b.hist, b.keys[1 - b.current], b.values[1 - b.current], kept, nullptr, 0, 0, t.delta);
+    placeTextEndsKernel<K><<<ceilDiv(texts, 256), 256, 0, stream>>>(t.endKeys, t.endValues, t.endDigits, t.byText, texts, kept,
+                                                                  b.keys[1 - b.current], b.values[1 - b.current]);
+    b.current = 1 - b.current;
+}
+
 // Class 0 of a batch needs no work: phase (a) ranks by (text, suffix), and the sorted sample
 // array is indexed by those global ranks.
 

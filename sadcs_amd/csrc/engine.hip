@@ -990,6 +990,12 @@ class Pipeline {
         uint32_t* walkKeys = nullptr;  // the walks of classes that read the text, evaluated once
         uint32_t* walkValues = nullptr;
         uint8_t* classDigits = nullptr;  // a batch's digit bytes between its passes over the text index
+        // A batch sorts each class by (text, character) in one pass when the per-text digit
+        // counts (3 words per text and digit) are small beside the class lists; else it adds
+        // digit passes over the text index.
+        const uint64_t textCells = static_cast<uint64_t>(m_segments) * kRadixBins;
+        const bool byText = kBatch && m_segments > 1 && textCells <= largest;
+        TextSortScratch<Key> ts;
         const uint64_t walk = largest + m_segments + static_cast<uint64_t>(m_listLength[0]);
         uint64_t textWalk = 0;
         for (int d = 1; d < kClasses; d++) {
@@ -999,6 +1005,12 @@ class Pipeline {
         }
         int status = SADCS_OK;
         auto cleanup = [&]() {
+            unscratch(ts.endDigits);
+            unscratch(ts.endValues);
+            unscratch(ts.endKeys);
+            unscratch(ts.delta);
+            unscratch(ts.byDigit);
+            unscratch(ts.byText);
             unscratch(classDigits);
             unscratch(walkValues);
             unscratch(walkKeys);
@@ -1016,10 +1028,19 @@ class Pipeline {
         // The histograms too go to the caller's buffer when it has room: every class list is live
         // now, the peak of a device-resident build.
         if (status == SADCS_OK) status = scratch(hist, radixHistWords(walk));
-        if (status == SADCS_OK) status = scratch(aggregates, radixAggregateBytes(walk) / sizeof(Sum1) + 1);
+        const size_t aggregateBytes = std::max(radixAggregateBytes(walk), byText ? scanScratchBytes<Sum1, kSumItems>(textCells) : size_t(0));
+        if (status == SADCS_OK) status = scratch(aggregates, aggregateBytes / sizeof(Sum1) + 1);
         if (status == SADCS_OK) status = scratch(walkKeys, std::max<uint64_t>(textWalk, 1));
         if (status == SADCS_OK) status = scratch(walkValues, std::max<uint64_t>(textWalk, 1));
-        if (kBatch && status == SADCS_OK) status = scratch(classDigits, std::max<uint64_t>(largest, 1));
+        if (kBatch && !byText && status == SADCS_OK) status = scratch(classDigits, std::max<uint64_t>(largest, 1));
+        if (byText) {
+            if (status == SADCS_OK) status = scratch(ts.byText, textCells);
+            if (status == SADCS_OK) status = scratch(ts.byDigit, textCells);
+            if (status == SADCS_OK) status = scratch(ts.delta, textCells);
+            if (status == SADCS_OK) status = scratch(ts.endKeys, m_segments);
+            if (status == SADCS_OK) status = scratch(ts.endValues, m_segments);
+            if (status == SADCS_OK) status = scratch(ts.endDigits, m_segments);
+        }
         if (status != SADCS_OK || m_dry) {
             cleanup();
             return status;
@@ -1048,7 +1069,7 @@ class Pipeline {
             SortBuffers<Key> sb;
             sb.hist = hist;
             sb.aggregates = aggregates;
-            const int extraPasses = kBatch ? (segmentBits + kRadixBits - 1) / kRadixBits : 0;
+            const int extraPasses = kBatch && !byText ? (segmentBits + kRadixBits - 1) / kRadixBits : 0;
             const int passes = 1 + extraPasses;
             // Buffer 1 is the destination (next keys, class d); buffer 0 the temporaries.
             sb.keys[0] = kBatch ? tmpKeys : keys[prev];
@@ -1062,7 +1083,14 @@ class Pipeline {
             auto segmentWidth = [&](int e) { return std::min(kRadixBits, segmentBits - e * kRadixBits); };
             sb.digits = classDigits;
             const int firstNext = extraPasses > 0 && classDigits ? 32 : -1;
-            if (fromText) {
+            if (byText) {
+                if (fromText) {
+                    materializeWalkKernel<decltype(source), Key><<<blocksFor(count), 256, 0, m_stream>>>(source, count, walkKeys, walkValues);
+                    sortClassByText<Key>(m_stream, sb, MaterializedWalk<Key, Texts>{walkKeys, walkValues, m_texts}, count, kept, m_segments, 24, ts);
+                } else {
+                    sortClassByText<Key>(m_stream, sb, source, count, kept, m_segments, 24, ts);
+                }
+            } else if (fromText) {
                 materializeWalkKernel<decltype(source), Key><<<blocksFor(count), 256, 0, m_stream>>>(source, count, walkKeys, walkValues);
                 radixPassDigits<Key>(m_stream, sb, MaterializedWalk<Key, Texts>{walkKeys, walkValues, m_texts}, count, 24, 8, kept,
                                      false, firstNext, firstNext >= 0 ? segmentWidth(0) : 0);

@@ -288,15 +288,36 @@ __device__ __forceinline__ uint32_t digitOf(K key, int shift, uint32_t mask) {
     return static_cast<uint32_t>(key >> shift) & mask;
 }
 
-// hist[d * tiles + tile] = number of kept items of the tile with digit d.
-template <class K, class Source>
-__global__ __launch_bounds__(kSortThreads) void radixUpsweepKernel(Source source, uint32_t count, int shift, uint32_t mask, uint32_t tiles, uint32_t* __restrict__ hist) {
+// The text of a batch item's key (text << 32 | characters).
+template <class K>
+__device__ __forceinline__ uint32_t textOf(K key) {
+    return static_cast<uint32_t>(static_cast<uint64_t>(key) >> 32);
+}
+
+constexpr uint32_t kTextSlots = 4;
+
+// hist[d * tiles + tile] = number of kept items of the tile with digit d. kTexts (items in
+// ascending text order, key = text << 32 | ...): also textCounts[text * 256 + d] += the tile's
+// items of that text and digit, through LDS for the tile's first kTextSlots texts.
+template <class K, class Source, bool kTexts = false>
+__global__ __launch_bounds__(kSortThreads) void radixUpsweepKernel(Source source, uint32_t count, int shift, uint32_t mask, uint32_t tiles, uint32_t* __restrict__ hist,
+                                                                   uint32_t* __restrict__ textCounts = nullptr) {
     __shared__ uint32_t counts[kSortWaves][kRadixBins];
+    __shared__ uint32_t textHist[kTexts ? kTextSlots : 1][kRadixBins];
+    __shared__ uint32_t textBase;
     const uint32_t wave = threadIdx.x / kWave;
     const uint32_t lane = threadIdx.x % kWave;
 
     for (uint32_t x = threadIdx.x; x < kSortWaves * kRadixBins; x += kSortThreads) {
         (&counts[0][0])[x] = 0;
+    }
+    if constexpr (kTexts) {
+        for (uint32_t x = threadIdx.x; x < kTextSlots * kRadixBins; x += kSortThreads) {
+            (&textHist[0][0])[x] = 0;
+        }
+        if (threadIdx.x == 0) {
+            textBase = ~0u;
+        }
     }
     __syncthreads();
 
@@ -313,12 +334,37 @@ __global__ __launch_bounds__(kSortThreads) void radixUpsweepKernel(Source source
         keys[j] = 0;
         valids[j] = source(clampIndex(i, count), keys[j], value) && i < count;
     }
+    if constexpr (kTexts) {
+        // The tile's first text: the least over its kept items.
+        uint32_t lo = ~0u;
+#pragma unroll
+        for (int j = 0; j < kSortItems; j++) {
+            lo = valids[j] ? min(lo, textOf(keys[j])) : lo;
+        }
+        for (int offset = kWave / 2; offset > 0; offset /= 2) {
+            lo = min(lo, static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo), offset)));
+        }
+        if (lane == 0 && lo != ~0u) {
+            atomicMin(&textBase, lo);
+        }
+        __syncthreads();
+    }
     // Counting needs no ranks: one LDS atomic add per item into the wave's own histogram (lanes
     // with equal digits serialise in the LDS; matching them first cost more VALU time).
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
         if (valids[j]) {
-            atomicAdd(&counts[wave][digitOf(keys[j], shift, mask)], 1u);
+            const uint32_t digit = digitOf(keys[j], shift, mask);
+            atomicAdd(&counts[wave][digit], 1u);
+            if constexpr (kTexts) {
+                const uint32_t text = textOf(keys[j]);
+                const uint32_t slot = text - textBase;
+                if (slot < kTextSlots) {
+                    atomicAdd(&textHist[slot][digit], 1u);
+                } else {
+                    atomicAdd(&textCounts[static_cast<size_t>(text) * kRadixBins + digit], 1u);
+                }
+            }
         }
     }
     __syncthreads();
@@ -329,6 +375,14 @@ __global__ __launch_bounds__(kSortThreads) void radixUpsweepKernel(Source source
             total += counts[w][d];
         }
         hist[static_cast<uint64_t>(d) * tiles + tile] = total;
+    }
+    if constexpr (kTexts) {
+        for (uint32_t x = threadIdx.x; x < kTextSlots * kRadixBins; x += kSortThreads) {
+            const uint32_t c = (&textHist[0][0])[x];
+            if (c != 0) {
+                atomicAdd(&textCounts[(static_cast<size_t>(textBase) + x / kRadixBins) * kRadixBins + x % kRadixBins], c);
+            }
+        }
     }
 }
 
@@ -382,9 +436,12 @@ __global__ __launch_bounds__(kSortThreads) void radixDigitUpsweepKernel(const ui
 // Writes beyond `limit` outputs are dropped (a source that keeps more items than planned). With
 // kDigits, every output item's next digit ((key >> nextShift) & nextMask) goes to digitsOut at its
 // output index, for radixDigitUpsweepKernel.
+// textDelta (a batch class sort by text and digit, sortClassByText): each item's output index moves by
+// textDelta[text * 256 + digit].
 template <class K, class Source, bool kDigits = false>
 __global__ __launch_bounds__(kSortThreads) void radixDownsweepKernel(Source source, uint32_t count, int shift, uint32_t mask, uint32_t tiles, const uint32_t* __restrict__ offsets, K* __restrict__ keysOut, uint32_t* __restrict__ valuesOut, uint32_t limit,
-                                                                     uint8_t* __restrict__ digitsOut = nullptr, int nextShift = 0, uint32_t nextMask = 0) {
+                                                                     uint8_t* __restrict__ digitsOut = nullptr, int nextShift = 0, uint32_t nextMask = 0,
+                                                                     const uint32_t* __restrict__ textDelta = nullptr) {
     __shared__ uint32_t waveCounts[kSortWaves][kRadixBins];
     __shared__ uint32_t digitStart[kRadixBins];
     __shared__ uint32_t globalStart[kRadixBins];
@@ -474,7 +531,10 @@ __global__ __launch_bounds__(kSortThreads) void radixDownsweepKernel(Source sour
     for (uint32_t x = threadIdx.x; x < kept; x += kSortThreads) {
         const K key = sortedKeys[x];
         const uint32_t digit = digitOf(key, shift, mask);
-        const uint32_t target = globalStart[digit] + (x - digitStart[digit]);
+        uint32_t target = globalStart[digit] + (x - digitStart[digit]);
+        if (textDelta) {
+            target += textDelta[static_cast<size_t>(textOf(key)) * kRadixBins + digit];
+        }
         if (target < limit) {
             keysOut[target] = key;
             valuesOut[target] = sortedValues[x];
