@@ -1174,7 +1174,8 @@ class Pipeline {
             if (!m_dry) {
                 const uint64_t threads = count * (kLineBytes / sizeof(uint4));
                 buildLinesKernel<<<static_cast<unsigned>((threads + 255) / 256), 256, 0, m_stream>>>(
-                    reinterpret_cast<const uint4*>(m_job.text), m_ranks, count, m_lines);
+                    reinterpret_cast<const uint4*>(m_job.text), m_ranks, count, m_lines,
+                    kBatch ? m_job.blockSegment : nullptr, kBatch ? m_job.deviceSegments : nullptr);
                 SADCS_HIP(hipGetLastError());
             }
             // Stream order keeps the build's reads of the ranks ahead of any reuse of their memory

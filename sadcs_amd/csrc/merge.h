@@ -92,7 +92,8 @@ inline uint64_t lineCount(uint64_t span) {
 
 // 8 threads per line, one 16-byte word each: words 0..3 text, 4..7 ranks. The text is 16-byte
 // aligned and readable to span + 63; the rank array holds 15 words past its last sample.
-__global__ void buildLinesKernel(const uint4* __restrict__ text, const uint32_t* __restrict__ ranks, uint64_t lines, uint4* __restrict__ out) {
+__global__ void buildLinesKernel(const uint4* __restrict__ text, const uint32_t* __restrict__ ranks, uint64_t lines, uint4* __restrict__ out,
+                                 const uint32_t* __restrict__ blockSegment, const Segment* __restrict__ segments) {
     const uint64_t x = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (x >= lines * 8) {
         return;
@@ -115,6 +116,13 @@ __global__ void buildLinesKernel(const uint4* __restrict__ text, const uint32_t*
         v.y = ranks[first + 1];
         v.z = t < 7 ? ranks[first + 2] : 0;
         v.w = t < 7 ? ranks[first + 3] : 0;
+        if (t == 7 && blockSegment) {
+            // A batch: the line's text and that text's end in the last two words (a line's 32
+            // positions lie in one text, which starts on a block boundary).
+            const uint32_t segment = blockSegment[(b * kLinePositions) >> kSegmentBlockLog];
+            v.z = segment;
+            v.w = segments[segment].end;
+        }
     }
     out[x] = v;
 }
@@ -967,7 +975,8 @@ __device__ __forceinline__ void loadRecordsFromLines(const MergeCtx& c, const Te
         for (int h = 0; h < kPairSlots; h++) {
             pos[h] = rowsIn[min(e0 + h * kPairs, size - 1)];
         }
-        uint4 quad[kPairSlots][3];
+        // A batch's rank lane loads all four rank quads: the last holds the line's text and its end.
+        uint4 quad[kPairSlots][Texts::kBatch ? 4 : 3];
 #pragma unroll
         for (int h = 0; h < kPairSlots; h++) {
             uint32_t p = pos[h];
@@ -980,10 +989,13 @@ __device__ __forceinline__ void loadRecordsFromLines(const MergeCtx& c, const Te
             }
             const uint32_t q = p & (kLinePositions - 1);
             const uint4* line = c.lines + (static_cast<uint64_t>(p / kLinePositions) << 3);
-            const uint4* from = textLane ? line + (q >> 4) : line + 4 + (coverBelow<32>(q) >> 2);
+            const uint4* from = textLane ? line + (q >> 4) : line + 4 + (Texts::kBatch ? 0 : coverBelow<32>(q) >> 2);
             quad[h][0] = from[0];
             quad[h][1] = from[1];
             quad[h][2] = from[2];
+            if constexpr (Texts::kBatch) {
+                quad[h][3] = textLane ? uint4{0, 0, 0, 0} : from[3];
+            }
         }
 #pragma unroll
         for (int h = 0; h < kPairSlots; h++) {
@@ -1016,6 +1028,30 @@ __device__ __forceinline__ void loadRecordsFromLines(const MergeCtx& c, const Te
                     const uint32_t b = offset - 1;
                     before[e] = p > 0 ? static_cast<uint8_t>(w[(b >> 3) & 1] >> ((b & 7) * 8)) : 0;
                 }
+            } else if constexpr (Texts::kBatch) {
+                // Ranks at words below(q) .. below(q) + 6 of the four quads (below(q) is 0 .. 7);
+                // words 14 and 15 are the text and its end.
+                const uint32_t o = coverBelow<32>(q);
+                uint32_t x[16];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    x[4 * k] = quad[h][k].x;
+                    x[4 * k + 1] = quad[h][k].y;
+                    x[4 * k + 2] = quad[h][k].z;
+                    x[4 * k + 3] = quad[h][k].w;
+                }
+#pragma unroll
+                for (int k = 0; k < CoverSet<32>::kSize; k++) {
+                    uint32_t r = x[k];
+#pragma unroll
+                    for (uint32_t j = 1; j <= CoverSet<32>::kSize; j++) {
+                        r = o == j ? x[k + j] : r;
+                    }
+                    recs.ranks[k * cap + e] = r;
+                }
+                recs.position[e] = p;
+                recs.tag[e] = (x[14] << 9) | min(x[15] - p, 32u);
+                indexA[e] = static_cast<uint16_t>(e);
             } else {
                 const uint32_t o = coverBelow<32>(q) & 3;
                 const uint4 r0 = quad[h][0], r1 = quad[h][1], r2 = quad[h][2];
@@ -1025,11 +1061,6 @@ __device__ __forceinline__ void loadRecordsFromLines(const MergeCtx& c, const Te
                     recs.ranks[k * cap + e] = o == 0 ? x[k] : o == 1 ? x[k + 1] : o == 2 ? x[k + 2] : x[k + 3];
                 }
                 recs.position[e] = p;
-                if constexpr (Texts::kBatch) {
-                    const uint32_t segment = texts.segmentOf(p);
-                    const uint32_t remaining = c.segments[segment].end - p;
-                    recs.tag[e] = (segment << 9) | min(remaining, 32u);
-                }
                 indexA[e] = static_cast<uint16_t>(e);
             }
         }
