@@ -438,15 +438,26 @@ struct MarkConsumer {
 // rank order, so their rank stores would scatter 4-byte writes over the whole rank array, each a
 // read-modify-write of a memory line; here one launch writes only the ranks in [first, last), a
 // range small enough for the partial lines to merge in the memory-side cache, and the launches
-// over consecutive ranges read the (coalesced) sorted sample again each time.
+// over consecutive ranges read the (coalesced) sorted sample again each time. Each thread loads
+// kItems entries before storing any: one load and store at a time left the launch latency-bound.
 template <int V>
-__global__ void scatterResolvedRanksKernel(const uint32_t* __restrict__ sortedSample, uint32_t samples, uint32_t first, uint32_t last, uint32_t* __restrict__ ranks) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < samples; i += gridDim.x * blockDim.x) {
-        const uint32_t p = sortedSample[i];
-        if (p != ~0u) {
-            const uint32_t t = packPosition<V>(p);
-            if (t >= first && t < last) {
-                ranks[t] = i;
+__global__ __launch_bounds__(256) void scatterResolvedRanksKernel(const uint32_t* __restrict__ sortedSample, uint32_t samples, uint32_t first, uint32_t last, uint32_t* __restrict__ ranks) {
+    constexpr uint32_t kItems = 16;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i0 = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i0 < samples; i0 += stride * kItems) {
+        uint32_t p[kItems];
+#pragma unroll
+        for (uint32_t k = 0; k < kItems; k++) {
+            const uint64_t i = i0 + k * stride;
+            p[k] = i < samples ? sortedSample[i] : ~0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kItems; k++) {
+            if (p[k] != ~0u) {
+                const uint32_t t = packPosition<V>(p[k]);
+                if (t >= first && t < last) {
+                    ranks[t] = static_cast<uint32_t>(i0 + k * stride);
+                }
             }
         }
     }
