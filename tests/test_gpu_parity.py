@@ -134,6 +134,23 @@ def test_batch_runs_among_repeats(gpu):
         assert (bwt, primary) == oracle.bwt_from_sa(texts[i], want), i
 
 
+def test_batch_alphabets(gpu):
+    """Batches whose texts' alphabets differ: with every byte value present somewhere (9-bit codes,
+    no per-text codes) and without (each text's first pass packs its own codes), against the
+    texts built alone."""
+    from sadcs_amd import workloads
+    rng = oracle.Rng(9)
+    narrow = [workloads.dna_like(60_000, 71).tobytes(), b"ab" * 20_000, rng.random_text(50_000, 2),
+              workloads.markov(80_000, 72).tobytes(), b"z" * 700]
+    wide = narrow + [bytes(range(256)) * 40 + rng.random_text(5000, 256)]
+    for name, texts in [("narrow", narrow), ("wide", wide)]:
+        print(f"batch {name}: {len(texts)} texts", flush=True)
+        for i, (sa, bwt, primary) in enumerate(gpu.gpu_batch_build(texts)):
+            want = expected_sa(texts[i])
+            assert np.array_equal(sa, want), (name, i)
+            assert (bwt, primary) == oracle.bwt_from_sa(texts[i], want), (name, i)
+
+
 @pytest.mark.skipif(not oracle.ref_available(), reason="needs the reference build in oracle/_ref")
 def test_large(gpu):
     rng = oracle.Rng(3)
