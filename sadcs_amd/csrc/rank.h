@@ -69,23 +69,31 @@ template <int V>
 __device__ __forceinline__ uint32_t unpackIndex(uint32_t t);
 
 // Pass 0: the i-th valid sample suffix of the batch, and its text's index.
+// The text of sample rank i in a batch: the last with rankBase <= i.
+__device__ __forceinline__ uint32_t textOfRank(const RankParams& rp, uint32_t i) {
+    uint32_t lo = 0, hi = rp.segments;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (rp.segmentTable[mid].rankBase <= i) {
+            lo = mid;
+        } else {
+            hi = mid;
+        }
+    }
+    return lo;
+}
+
+// The first pass's element i: its compact index t and its text. blockText/blockTextEnd: the text
+// of the workgroup's first element and the rank where the next text starts; elements before that
+// skip the search.
 template <int V, class Texts>
-__device__ __forceinline__ void firstPassEntry(const RankParams& rp, uint32_t i, uint32_t& t, uint32_t& segment) {
+__device__ __forceinline__ void firstPassEntry(const RankParams& rp, uint32_t i, uint32_t& t, uint32_t& segment, uint32_t blockText, uint32_t blockTextEnd) {
     if constexpr (!Texts::kBatch) {
         t = i;
         segment = 0;
     } else {
-        uint32_t lo = 0, hi = rp.segments;  // last segment with rankBase <= i
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) / 2;
-            if (rp.segmentTable[mid].rankBase <= i) {
-                lo = mid;
-            } else {
-                hi = mid;
-            }
-        }
-        segment = lo;
-        const Segment s = rp.segmentTable[lo];
+        segment = i < blockTextEnd ? blockText : textOfRank(rp, i);
+        const Segment s = rp.segmentTable[segment];
         t = packPosition<V>(s.start) + (i - s.rankBase);
     }
 }
@@ -97,8 +105,26 @@ __device__ __forceinline__ void firstPassEntry(const RankParams& rp, uint32_t i,
 template <int V, class Texts, bool kWideCodes>
 __global__ __launch_bounds__(256) void makePassKeysKernel(RankParams rp, Texts texts, PassSpec spec, CodeTable codes, bool first, const uint32_t* __restrict__ listT, const uint32_t* __restrict__ listOrdinal, uint32_t u, uint64_t* __restrict__ keysOut, uint32_t* __restrict__ valuesOut, uint8_t* __restrict__ digitsOut) {
     __shared__ uint8_t code[256];
+    __shared__ uint8_t blockCodes[256];  // a batch's first pass: the codes of the workgroup's first text
+    __shared__ uint32_t blockText, blockTextEnd;
     for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) {
         code[x] = codes.code[x];
+    }
+    if constexpr (Texts::kBatch) {
+        if (first) {
+            // The workgroup's elements are consecutive ranks, mostly of one text: one search.
+            if (threadIdx.x == 0) {
+                const uint32_t text = textOfRank(rp, min(blockIdx.x * blockDim.x, u - 1));
+                blockText = text;
+                blockTextEnd = text + 1 < rp.segments ? rp.segmentTable[text + 1].rankBase : ~0u;
+            }
+            __syncthreads();
+            if (rp.textCodes) {
+                for (uint32_t x = threadIdx.x; x < 256; x += blockDim.x) {
+                    blockCodes[x] = rp.textCodes[static_cast<size_t>(blockText) * 256 + x];
+                }
+            }
+        }
     }
     __syncthreads();
 
@@ -109,7 +135,7 @@ __global__ __launch_bounds__(256) void makePassKeysKernel(RankParams rp, Texts t
 
     uint32_t t, ordinal;
     if (first) {
-        firstPassEntry<V, Texts>(rp, i, t, ordinal);
+        firstPassEntry<V, Texts>(rp, i, t, ordinal, Texts::kBatch ? blockText : 0u, Texts::kBatch ? blockTextEnd : 0u);
     } else {
         t = listT[i];
         ordinal = listOrdinal[i];
@@ -125,18 +151,22 @@ __global__ __launch_bounds__(256) void makePassKeysKernel(RankParams rp, Texts t
         // Texts of a batch never compare, so each packs its own codes as densely as they allow
         // (a DNA text 3 bits per character among texts of 96 symbols), left-aligned in the
         // subkey. The ordinal of a first-pass element is its text.
-        const uint8_t* table = rp.textCodes + static_cast<size_t>(ordinal) * 256;
+        // The workgroup's first text reads its codes from LDS, others from the global table.
+        const uint8_t* table = ordinal == blockText ? blockCodes : rp.textCodes + static_cast<size_t>(ordinal) * 256;
         const int bits = rp.textShape[2 * ordinal], chars = rp.textShape[2 * ordinal + 1];
         const uint32_t q = p + spec.offset;
         const uint32_t remaining = q < end ? end - q : 0;
-        uint64_t w = 0;
-        for (int c = 0; c < chars; c++) {
-            if ((c & 7) == 0) {
-                w = static_cast<uint32_t>(c) < remaining ? load8(rp.text, q + c) : 0;
+        // Eight characters per text word, their code lookups independent of each other.
+        for (int c0 = 0; c0 < chars; c0 += 8) {
+            uint64_t w = static_cast<uint32_t>(c0) < remaining ? load8(rp.text, q + c0) : 0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int c = c0 + j;
+                const uint32_t byte = static_cast<uint32_t>(w >> 56);
+                w <<= 8;
+                const uint32_t ch = static_cast<uint32_t>(c) < remaining ? table[byte] : 0u;
+                subkey = c < chars ? (subkey << bits) | ch : subkey;
             }
-            const uint32_t byte = static_cast<uint32_t>(w >> 56);
-            w <<= 8;
-            subkey = (subkey << bits) | (static_cast<uint32_t>(c) < remaining ? table[byte] : 0u);
         }
         subkey <<= spec.subkeyBits - chars * bits;
     } else {
