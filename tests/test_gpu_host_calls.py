@@ -93,3 +93,24 @@ def test_batch_slices_cut_across_texts(gpu):
         want_sa, want_bwt, want_primary = gpu.gpu_build(t)
         assert np.array_equal(sa, want_sa), i
         assert bwt == want_bwt and primary == want_primary, i
+
+
+def test_two_slices_odd_length(gpu):
+    """Just over 2^24 rows (two slices, the second short) and an odd length: SA, BWT and primary
+    from a host call equal the device-resident build's, for both outputs and for each alone."""
+    from sadcs_amd import workloads
+    n = (1 << 24) + 12345
+    text = workloads.dna_like(n, 31, padding=64)
+    d_text = gpu.DeviceBuffer.from_host(text)
+    d_sa = gpu.DeviceBuffer(4 * n)
+    d_bwt = gpu.DeviceBuffer(n)
+    with gpu.GpuContext(n) as ctx:
+        primary = gpu.gpu_build_device(ctx, d_text.ptr, n, d_sa.ptr, d_bwt.ptr)
+    assert gpu.gpu_check_device(d_text.ptr, n, d_sa.ptr, d_bwt.ptr, primary) == (0, -1)
+    sa, bwt = d_sa.to_host(np.int32), d_bwt.to_host(np.uint8).tobytes()
+    host = text[:n].copy()
+    got_sa, got_bwt, got_primary = gpu.gpu_build(host)
+    assert np.array_equal(got_sa, sa)
+    assert got_bwt == bwt and got_primary == primary
+    assert np.array_equal(gpu.gpu_suffix_array(host), sa)
+    assert gpu.gpu_bwt(host) == (bwt, primary)
