@@ -612,11 +612,13 @@ constexpr int kLoadSlots = 3;  // records each thread gathers at once (default t
 template <int V, bool kBatch>
 struct LdsRecords {
     static constexpr int kSize = CoverSet<V>::kSize;
+    // A batch keeps each record's position and tag side by side, so that a merge step reads both
+    // with one 8-byte LDS load.
+    static constexpr uint32_t kPositionStride = kBatch ? 2 : 1;
     uint32_t capacity;
     uint64_t* words;     // [kRecordWords][capacity]
-    uint32_t* position;  // [capacity]
+    uint32_t* position;  // [capacity], or (position, tag) pairs [capacity][2] for a batch
     uint32_t* ranks;     // [kSize][capacity]
-    uint32_t* tag;       // [capacity], batch only
 
     static constexpr size_t bytesPerRecord() {
         return 8 * kRecordWords + 4 + 4 * kSize + (kBatch ? 4 : 0);
@@ -626,22 +628,35 @@ struct LdsRecords {
         capacity = cap;
         words = reinterpret_cast<uint64_t*>(base);
         position = reinterpret_cast<uint32_t*>(words + static_cast<size_t>(kRecordWords) * cap);
-        ranks = position + cap;
-        tag = ranks + static_cast<size_t>(kSize) * cap;
+        ranks = position + static_cast<size_t>(kPositionStride) * cap;
     }
 
-    __device__ __forceinline__ void store(uint32_t e, const MergeRecord<V, kBatch>& r) {
+    __device__ __forceinline__ uint32_t positionOf(uint32_t e) const {
+        return position[e * kPositionStride];
+    }
+    __device__ __forceinline__ uint32_t tagOf(uint32_t e) const {  // batch only
+        return position[e * 2 + 1];
+    }
+    __device__ __forceinline__ void setPosition(uint32_t e, uint32_t p) const {
+        position[e * kPositionStride] = p;
+    }
+    __device__ __forceinline__ void setTag(uint32_t e, uint32_t t) const {  // batch only
+        position[e * 2 + 1] = t;
+    }
+
+    __device__ __forceinline__ void store(uint32_t e, const MergeRecord<V, kBatch>& r) const {
 #pragma unroll
         for (int w = 0; w < kRecordWords; w++) {
             words[w * capacity + e] = r.words[w];
         }
-        position[e] = r.position;
+        if constexpr (kBatch) {
+            reinterpret_cast<uint2*>(position)[e] = uint2{r.position, r.tag};
+        } else {
+            position[e] = r.position;
+        }
 #pragma unroll
         for (int s = 0; s < kSize; s++) {
             ranks[s * capacity + e] = r.ranks[s];
-        }
-        if constexpr (kBatch) {
-            tag[e] = r.tag;
         }
     }
 };
@@ -675,9 +690,15 @@ struct HeadKey {
 template <int V, bool kBatch>
 __device__ __forceinline__ HeadKey<kBatch> headKey(const LdsRecords<V, kBatch>& R, uint32_t x) {
     HeadKey<kBatch> k;
-    k.position = R.position[x];
+    if constexpr (kBatch) {
+        const uint2 pt = reinterpret_cast<const uint2*>(R.position)[x];
+        k.position = pt.x;
+        k.tag = pt.y;
+    } else {
+        k.position = R.position[x];
+        k.tag = 0;
+    }
     k.word = R.words[x];
-    k.tag = kBatch ? R.tag[x] : 0;
     return k;
 }
 
@@ -741,14 +762,14 @@ __device__ __forceinline__ bool lessLds(const MergeCtx& c, const TieTables& t, c
     } else {
         uint32_t ra, rb;
         if constexpr (Texts::kBatch) {
-            const uint32_t ta = R.tag[x], tb = R.tag[y];
+            const uint32_t ta = R.tagOf(x), tb = R.tagOf(y);
             if ((ta >> 9) != (tb >> 9)) {
                 return (ta >> 9) < (tb >> 9);
             }
             ra = ta & 0x1FF;
             rb = tb & 0x1FF;
         }
-        const uint32_t i = R.position[x], j = R.position[y];
+        const uint32_t i = R.positionOf(x), j = R.positionOf(y);
         if constexpr (!Texts::kBatch) {
             ra = min(c.n - i, static_cast<uint32_t>(V));
             rb = min(c.n - j, static_cast<uint32_t>(V));
@@ -921,11 +942,11 @@ __device__ __forceinline__ void writeTile(const MergeCtx& c, const ListSet& list
     const int lc = lists.count;
     for (uint32_t r = threadIdx.x; r < size; r += kThreads) {
         const uint32_t e = order[r];
-        const uint32_t p = recs.position[e];
+        const uint32_t p = recs.positionOf(e);
         const uint32_t row = outBase + r;
         uint32_t start = 0, rowBase = 0, segment = 0;
         if constexpr (Texts::kBatch) {
-            segment = recs.tag[e] >> 9;
+            segment = recs.tagOf(e) >> 9;
             start = c.segments[segment].start;
             rowBase = c.segments[segment].rowBase;
         }
@@ -1049,8 +1070,8 @@ __device__ __forceinline__ void loadRecordsFromLines(const MergeCtx& c, const Te
                     }
                     recs.ranks[k * cap + e] = r;
                 }
-                recs.position[e] = p;
-                recs.tag[e] = (x[14] << 9) | min(x[15] - p, 32u);
+                recs.setPosition(e, p);
+                recs.setTag(e, (x[14] << 9) | min(x[15] - p, 32u));
                 indexA[e] = static_cast<uint16_t>(e);
             } else {
                 const uint32_t o = coverBelow<32>(q) & 3;
@@ -1060,7 +1081,7 @@ __device__ __forceinline__ void loadRecordsFromLines(const MergeCtx& c, const Te
                 for (int k = 0; k < CoverSet<32>::kSize; k++) {
                     recs.ranks[k * cap + e] = o == 0 ? x[k] : o == 1 ? x[k + 1] : o == 2 ? x[k + 2] : x[k + 3];
                 }
-                recs.position[e] = p;
+                recs.setPosition(e, p);
                 indexA[e] = static_cast<uint16_t>(e);
             }
         }
