@@ -205,6 +205,12 @@ int sadcs_hip_server_create(const sadcs_server_params* params, sadcs_server** se
 /* Copies the text and queues it; returns the request id (>= 0) or minus a status. */
 int64_t sadcs_hip_server_submit(sadcs_server* server, const uint8_t* text, int64_t n);
 
+/* As sadcs_hip_server_submit, with the outputs written straight into the caller's buffers (sa: n
+ * int32, bwt: n bytes, either may be null) when the request completes, instead of being held for
+ * wait to copy; the buffers must stay valid until wait returns for this id, and wait then ignores
+ * its own sa and bwt. No counterpart in the reference: the server is this library's. */
+int64_t sadcs_hip_server_submit_into(sadcs_server* server, const uint8_t* text, int64_t n, int32_t* sa, uint8_t* bwt);
+
 /* Blocks until the request is done and copies its outputs (sa: n int32, bwt: n bytes, either may be
  * null). latency: submission to completion; queued: submission to admission; batch_size: texts in its batch. */
 int sadcs_hip_server_wait(sadcs_server* server, int64_t id, int32_t* sa, uint8_t* bwt, int32_t* primary,

@@ -97,6 +97,8 @@ def _load():
     lib.sadcs_hip_server_create.restype = ctypes.c_int
     lib.sadcs_hip_server_submit.argtypes = [P, P, i64]
     lib.sadcs_hip_server_submit.restype = i64
+    lib.sadcs_hip_server_submit_into.argtypes = [P, P, i64, P, P]
+    lib.sadcs_hip_server_submit_into.restype = i64
     lib.sadcs_hip_server_wait.argtypes = [P, i64, P, P, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
     lib.sadcs_hip_server_wait.restype = ctypes.c_int
@@ -134,7 +136,7 @@ EXPORTED_SYMBOLS = (
     "sadcs_hip_ctx_destroy", "sadcs_hip_build", "sadcs_hip_build_device", "sadcs_hip_rank_sample",
     "sadcs_hip_alloc_pinned", "sadcs_hip_free_pinned", "sadcs_hip_last_error", "sadcs_hip_batch_build",
     "sadcs_hip_batch_required_bytes", "sadcs_hip_batch_layout", "sadcs_hip_batch_build_device",
-    "sadcs_hip_server_create", "sadcs_hip_server_submit", "sadcs_hip_server_wait", "sadcs_hip_server_counters",
+    "sadcs_hip_server_create", "sadcs_hip_server_submit", "sadcs_hip_server_submit_into", "sadcs_hip_server_wait", "sadcs_hip_server_counters",
     "sadcs_hip_server_destroy", "sadcs_hip_ctx_create_batch", "sadcs_hip_device_required_bytes", "sadcs_hip_tune", "sadcs_hip_device_alloc", "sadcs_hip_device_free", "sadcs_hip_memcpy",
     "sadcs_hip_memset", "sadcs_hip_synchronize", "sadcs_hip_check_device",
 )
@@ -357,28 +359,34 @@ class Server:
         self.handle = handle
         self.suffix_array = suffix_array
         self.bwt = bwt
-        self._lengths = {}
+        self._outputs = {}  # id -> (n, sa, bwt): arrays the server writes into, alive until wait
 
     def submit(self, text):
+        """Queues a text; its outputs go straight into arrays allocated here (submit_into)."""
         t = _as_bytes(text)
-        rid = lib.sadcs_hip_server_submit(self.handle, t.ctypes.data if t.size else None, t.size)
+        n = t.size
+        sa = np.empty(max(n, 1), dtype=np.int32) if self.suffix_array else None
+        b = np.empty(max(n, 1), dtype=np.uint8) if self.bwt else None
+        rid = lib.sadcs_hip_server_submit_into(self.handle, t.ctypes.data if n else None, n,
+                                               sa.ctypes.data if sa is not None else None,
+                                               b.ctypes.data if b is not None else None)
         if rid < 0:
             _check(-rid)
-        self._lengths[rid] = t.size
+        self._outputs[rid] = (n, sa, b)
         return rid
 
     def wait(self, rid):
         """Returns (sa, bwt bytes, primary, latency seconds, queued seconds, batch size)."""
-        n = self._lengths.pop(rid)
-        sa = np.empty(max(n, 1), dtype=np.int32) if self.suffix_array else None
-        b = np.empty(max(n, 1), dtype=np.uint8) if self.bwt else None
+        n, sa, b = self._outputs[rid]
         primary = ctypes.c_int32(-1)
         latency = ctypes.c_double(0)
         queued = ctypes.c_double(0)
         batch = ctypes.c_int32(0)
-        _check(lib.sadcs_hip_server_wait(self.handle, rid, sa.ctypes.data if sa is not None else None,
-                                         b.ctypes.data if b is not None else None, ctypes.byref(primary),
-                                         ctypes.byref(latency), ctypes.byref(queued), ctypes.byref(batch)))
+        try:
+            _check(lib.sadcs_hip_server_wait(self.handle, rid, None, None, ctypes.byref(primary),
+                                             ctypes.byref(latency), ctypes.byref(queued), ctypes.byref(batch)))
+        finally:
+            del self._outputs[rid]
         return (sa[:n] if sa is not None else None, b[:n].tobytes() if b is not None else None, primary.value,
                 latency.value, queued.value, batch.value)
 
@@ -388,15 +396,21 @@ class Server:
         return {"completed": c[0], "batches": c[1], "peak_pool_bytes": c[2], "budget": c[3], "max_concurrent_batches": c[4]}
 
     def close(self):
+        """Finishes the queued requests (they write into this object's arrays), then stops."""
         if self.handle:
             lib.sadcs_hip_server_destroy(self.handle)
             self.handle = None
+        self._outputs.clear()
 
     def __enter__(self):
         return self
 
     def __exit__(self, *exc):
         self.close()
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self.close()
 
 
 class DeviceBuffer:

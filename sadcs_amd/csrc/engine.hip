@@ -2734,9 +2734,11 @@ struct HostArray {
 
 struct ServerRequest {
     int64_t id = 0;
-    std::vector<uint8_t> text;
+    HostArray<uint8_t> text;
     HostArray<int32_t> sa;
     HostArray<uint8_t> bwt;
+    int32_t* outSA = nullptr;   // the caller's buffers, written at completion (submit_into);
+    uint8_t* outBWT = nullptr;  //   otherwise the outputs are held here until wait copies them
     int32_t primary = -1;
     int status = SADCS_OK;
     std::string error;
@@ -2832,9 +2834,14 @@ class Server {
         m_stopped = true;
     }
 
-    int64_t submit(const uint8_t* text, int64_t n) {
+    int64_t submit(const uint8_t* text, int64_t n, int32_t* outSA = nullptr, uint8_t* outBWT = nullptr) {
         auto r = std::make_shared<ServerRequest>();
-        r->text.assign(text, text + n);
+        r->text.allocate(static_cast<size_t>(n));
+        if (n > 0) {
+            HostCopier::instance().copy(r->text.data.get(), text, static_cast<size_t>(n));
+        }
+        r->outSA = m_wantSA ? outSA : nullptr;
+        r->outBWT = m_wantBWT ? outBWT : nullptr;
         r->arrival = std::chrono::steady_clock::now();
         std::lock_guard<std::mutex> lock(m_mutex);
         r->id = m_nextId++;
@@ -2911,7 +2918,7 @@ class Server {
     size_t planOf(const std::vector<std::shared_ptr<ServerRequest>>& batch) {
         std::vector<int64_t> lengths;
         for (auto& r : batch) {
-            lengths.push_back(static_cast<int64_t>(r->text.size()));
+            lengths.push_back(static_cast<int64_t>(r->text.size));
         }
         const Layout layout = layoutFor(m_params, lengths.data(), static_cast<int32_t>(lengths.size()), batch.size() > 1);
         sadcs_ctx planner;
@@ -2935,14 +2942,14 @@ class Server {
         tooLarge = false;
         uint64_t total = 0;
         for (auto& r : m_pending) {
-            if (r->text.size() <= 1) {
+            if (r->text.size <= 1) {
                 continue;  // answered by the dispatcher directly
             }
-            if (!batch.empty() && (total + r->text.size() > m_maxBatchBytes || static_cast<int>(batch.size()) >= m_maxBatchTexts)) {
+            if (!batch.empty() && (total + r->text.size > m_maxBatchBytes || static_cast<int>(batch.size()) >= m_maxBatchTexts)) {
                 break;
             }
             batch.push_back(r);
-            total += r->text.size() + kSegmentAlign;
+            total += r->text.size + kSegmentAlign;
         }
         while (!batch.empty()) {
             const size_t plan = planOf(batch);
@@ -2976,16 +2983,16 @@ class Server {
             // Texts of length 0 or 1 need no device work.
             for (auto it = m_pending.begin(); it != m_pending.end();) {
                 auto& r = *it;
-                if (r->text.size() <= 1) {
+                if (r->text.size <= 1) {
                     r->admitted = std::chrono::steady_clock::now();
-                    if (r->text.size() == 1) {
-                        r->sa.allocate(m_wantSA ? 1 : 0);
-                        r->bwt.allocate(m_wantBWT ? 1 : 0);
+                    if (r->text.size == 1) {
+                        r->sa.allocate(m_wantSA && !r->outSA ? 1 : 0);
+                        r->bwt.allocate(m_wantBWT && !r->outBWT ? 1 : 0);
                         if (m_wantSA) {
-                            r->sa.data[0] = 0;
+                            (r->outSA ? r->outSA : r->sa.data.get())[0] = 0;
                         }
                         if (m_wantBWT) {
-                            r->bwt.data[0] = r->text[0];
+                            (r->outBWT ? r->outBWT : r->bwt.data.get())[0] = r->text.data[0];
                         }
                         r->primary = 1;
                     } else {
@@ -3090,9 +3097,9 @@ class Server {
         std::vector<const uint8_t*> texts;
         uint64_t total = 0;
         for (auto& r : batch) {
-            lengths.push_back(static_cast<int64_t>(r->text.size()));
-            texts.push_back(r->text.data());
-            total += r->text.size();
+            lengths.push_back(static_cast<int64_t>(r->text.size));
+            texts.push_back(r->text.data.get());
+            total += r->text.size;
         }
         const Layout layout = layoutFor(m_params, lengths.data(), static_cast<int32_t>(lengths.size()), batch.size() > 1);
         // Outputs land in the worker's pinned buffer (one DMA each, no staging copies), grown to
@@ -3139,22 +3146,26 @@ class Server {
         uint64_t offset = 0;
         for (size_t i = 0; i < batch.size(); i++) {
             auto& r = batch[i];
-            const size_t n = r->text.size();
+            const size_t n = r->text.size;
             r->status = status;
             r->error = error;
             if (status == SADCS_OK) {
                 if (m_wantSA) {
-                    r->sa.allocate(n);
-                    HostCopier::instance().copy(r->sa.data.get(), sa + offset, n * sizeof(int32_t));
+                    if (!r->outSA) {
+                        r->sa.allocate(n);
+                    }
+                    HostCopier::instance().copy(r->outSA ? r->outSA : r->sa.data.get(), sa + offset, n * sizeof(int32_t));
                 }
                 if (m_wantBWT) {
-                    r->bwt.allocate(n);
-                    HostCopier::instance().copy(r->bwt.data.get(), bwt + offset, n);
+                    if (!r->outBWT) {
+                        r->bwt.allocate(n);
+                    }
+                    HostCopier::instance().copy(r->outBWT ? r->outBWT : r->bwt.data.get(), bwt + offset, n);
                     r->primary = primaries[i];
                 }
             }
             offset += n;
-            std::vector<uint8_t>().swap(r->text);
+            r->text.allocate(0);
         }
     }
 
@@ -3219,6 +3230,14 @@ int64_t sadcs_hip_server_submit(sadcs_server* server, const uint8_t* text, int64
         return -SADCS_FAILED;
     }
     return server->server->submit(text, n);
+}
+
+int64_t sadcs_hip_server_submit_into(sadcs_server* server, const uint8_t* text, int64_t n, int32_t* sa, uint8_t* bwt) {
+    if (!server || n < 0 || n > INT32_MAX - 1024 || (n > 0 && !text)) {
+        setError("invalid submission");
+        return -SADCS_FAILED;
+    }
+    return server->server->submit(text, n, sa, bwt);
 }
 
 int sadcs_hip_server_wait(sadcs_server* server, int64_t id, int32_t* sa, uint8_t* bwt, int32_t* primary, double* latency_seconds,
