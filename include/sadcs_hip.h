@@ -207,8 +207,10 @@ int64_t sadcs_hip_server_submit(sadcs_server* server, const uint8_t* text, int64
 
 /* As sadcs_hip_server_submit, with the outputs written straight into the caller's buffers (sa: n
  * int32, bwt: n bytes, either may be null) when the request completes, instead of being held for
- * wait to copy; the buffers must stay valid until wait returns for this id, and wait then ignores
- * its own sa and bwt. No counterpart in the reference: the server is this library's. */
+ * wait to copy; the buffers must stay valid until wait returns for this id. An output the server
+ * makes (want_sa, want_bwt) whose buffer is null here is held as after sadcs_hip_server_submit,
+ * and wait copies it into its own sa or bwt argument; wait writes nothing for an output given here.
+ * No counterpart in the reference: the server is this library's. */
 int64_t sadcs_hip_server_submit_into(sadcs_server* server, const uint8_t* text, int64_t n, int32_t* sa, uint8_t* bwt);
 
 /* Blocks until the request is done and copies its outputs (sa: n int32, bwt: n bytes, either may be

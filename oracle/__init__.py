@@ -238,3 +238,30 @@ def ref_check_bwt(text, bwt, primary):
     p, n = _padded(text)
     b = np.frombuffer(bytes(bwt), dtype=np.uint8) if n else np.zeros(1, dtype=np.uint8)
     return ref_lib().ref_check_bwt(p.ctypes.data, n, b.ctypes.data, primary)
+
+
+def ref_outputs(texts, cover=128, workers=None, bwt=True):
+    """The reference build's suffix array of every text, and the BWT read off it by bwt_from_sa,
+    computed by a pool of threads that each run the reference single-threaded (texts are
+    independent; the reference's output does not depend on its thread count). Returns a list of
+    (sa, bwt bytes, primary) in the order of texts (bwt and primary None when bwt is False)."""
+    import concurrent.futures
+
+    lib = ref_lib()
+    if lib is None:
+        raise RuntimeError("the reference build (oracle/_ref) is missing")
+    # omp_set_num_threads sets the calling thread's own team size, so each pool thread runs the
+    # reference on one thread while other callers keep theirs.
+    workers = workers or min(16, os.cpu_count() or 1)
+    order = sorted(range(len(texts)), key=lambda i: -len(texts[i]))
+
+    def one(i):
+        lib.ref_set_threads(1)
+        sa = ref_suffix_array(texts[i], cover)
+        return i, (sa, *(bwt_from_sa(texts[i], sa) if bwt else (None, None)))
+
+    out = [None] * len(texts)
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for i, result in pool.map(one, order):
+            out[i] = result
+    return out

@@ -279,6 +279,22 @@ void sadcs_gen_periodic(uint64_t seed, uint8_t* out, int64_t n) {
     }
 }
 
+// The batch configuration's texts, text i (of lengths[i] bytes, seed base * 1000003 + i) at
+// out + offsets[i]: DNA-like for even i, Markov for odd i. Texts are generated in parallel, each on
+// one thread (the nested chunk loops run serially), and every text is the same bytes as the
+// single-text generator gives for its seed and length.
+void sadcs_gen_batch(uint64_t base, int32_t count, const int64_t* lengths, const int64_t* offsets, uint8_t* out) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t i = 0; i < count; i++) {
+        const uint64_t seed = base * 1000003ull + static_cast<uint64_t>(i);
+        if (i % 2 == 0) {
+            sadcs_gen_dna_like(seed, out + offsets[i], lengths[i], nullptr);
+        } else {
+            sadcs_gen_markov(seed, out + offsets[i], lengths[i]);
+        }
+    }
+}
+
 // Lengths floor(exp(U[ln lo, ln hi])).
 void sadcs_gen_log_uniform_lengths(uint64_t seed, int32_t count, int64_t lo, int64_t hi, int64_t* lengths) {
     Rng rng(derive(seed, ~4ull));

@@ -30,6 +30,8 @@ _lib.sadcs_gen_log_uniform_lengths.argtypes = [_u64, ctypes.c_int32, _i64, _i64,
 _lib.sadcs_gen_log_uniform_lengths.restype = None
 _lib.sadcs_gen_serving_requests.argtypes = [_u64, ctypes.c_int32, ctypes.c_double, ctypes.c_double, _i64, _P, _P]
 _lib.sadcs_gen_serving_requests.restype = None
+_lib.sadcs_gen_batch.argtypes = [_u64, ctypes.c_int32, _P, _P, _P]
+_lib.sadcs_gen_batch.restype = None
 
 MiB = 1 << 20
 
@@ -89,14 +91,24 @@ def batch_lengths(count=4096, seed=4, lo=1024, hi=4 * MiB):
     return lengths
 
 
-def batch_texts(count=4096, seed=4, lo=1024, hi=4 * MiB):
-    """C4: texts of log-uniform lengths; even index DNA-like, odd index Markov."""
+def batch_buffer(count=4096, seed=4, lo=1024, hi=4 * MiB, offsets=None, padding=64):
+    """C4 in one buffer: (buffer, offsets, lengths), text i at buffer[offsets[i]:offsets[i] + lengths[i]]
+    (packed back to back unless offsets are given; the buffer ends with `padding` zero bytes).
+    Texts are generated in parallel; each equals batch_texts' text i."""
     lengths = batch_lengths(count, seed, lo, hi)
-    texts = []
-    for i, n in enumerate(lengths):
-        gen = dna_like if i % 2 == 0 else markov
-        texts.append(gen(int(n), seed * 1_000_003 + i))
-    return texts
+    if offsets is None:
+        offsets = np.concatenate(([0], np.cumsum(lengths)[:-1])).astype(np.int64)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    span = int((offsets + lengths).max()) if count else 0
+    buffer = np.zeros(span + padding, dtype=np.uint8)
+    _lib.sadcs_gen_batch(seed, count, lengths.ctypes.data, offsets.ctypes.data, buffer.ctypes.data)
+    return buffer, offsets, lengths
+
+
+def batch_texts(count=4096, seed=4, lo=1024, hi=4 * MiB):
+    """C4: texts of log-uniform lengths; even index DNA-like (seed * 1000003 + i), odd index Markov."""
+    buffer, offsets, lengths = batch_buffer(count, seed, lo, hi)
+    return [buffer[o:o + n] for o, n in zip(offsets.tolist(), lengths.tolist())]
 
 
 def serving_requests(count, seed=5, median=256 * 1024, sigma=1.5, cap=64 * MiB):

@@ -375,3 +375,31 @@ def test_device_outputs_at_any_offset(gpu):
         bwt = d_bwt.to_host(np.uint8)[bwt_offset:bwt_offset + n].tobytes()
         assert np.array_equal(sa, want[0]), f"offsets {sa_offset}, {bwt_offset}: SA"
         assert (bwt, primary) == (want[1], want[2]), f"offsets {sa_offset}, {bwt_offset}: BWT"
+
+
+def test_serving_submit_into_with_one_output(gpu):
+    """submit_into with only the SA buffer on a server that makes both outputs: the SA arrives in
+    the caller's buffer, the BWT is held and wait copies it into wait's own buffer (and wait leaves
+    its SA buffer alone)."""
+    import ctypes
+    lib = gpu.lib
+    texts = [oracle.Rng(31).random_text(n, 4) for n in (70_000, 3000, 1)]
+    with gpu.Server(workers=1) as server:
+        held = []
+        for text in texts:
+            t = np.frombuffer(text, dtype=np.uint8)
+            sa = np.full(t.size, -7, dtype=np.int32)
+            rid = lib.sadcs_hip_server_submit_into(server.handle, t.ctypes.data, t.size, sa.ctypes.data, None)
+            assert rid >= 0
+            held.append((rid, t, sa))
+        for rid, t, sa in held:
+            wait_sa = np.full(t.size, -9, dtype=np.int32)
+            wait_bwt = np.zeros(t.size, dtype=np.uint8)
+            primary = ctypes.c_int32(-1)
+            status = lib.sadcs_hip_server_wait(server.handle, rid, wait_sa.ctypes.data, wait_bwt.ctypes.data,
+                                               ctypes.byref(primary), None, None, None)
+            assert status == 0, gpu.gpu_last_error()
+            want = oracle.naive_sa(t.tobytes())
+            assert np.array_equal(sa, want)
+            assert (wait_bwt.tobytes(), primary.value) == oracle.bwt_from_sa(t.tobytes(), want)
+            assert (wait_sa == -9).all(), "wait wrote an output that was given to submit_into"
