@@ -288,11 +288,11 @@ namespace {
             case SADCS_CHECK_PASS:
                 return {true, -1, nullptr};
             case SADCS_CHECK_RANGE:
-                return {false, result[1], "entry out of range"};
+                return {false, result[1], "SA[index] is not a position of the text"};
             case SADCS_CHECK_DUPLICATE:
-                return {false, result[1], "duplicate entry"};
+                return {false, result[1], "SA[index] repeats another entry"};
             default:
-                return {false, result[1], "SA[index] and SA[index + 1] out of order"};
+                return {false, result[1], "suffix SA[index] is not smaller than suffix SA[index + 1]"};
         }
     }
 
