@@ -17,9 +17,12 @@ Printed as one JSON line, with:
   host_call: the same 1 GiB text through the host-memory API (pageable text in, SA and BWT out,
     PCIe included; the download overlaps the end of the merge), best of 3 calls, outputs compared
     with the device-resident build's - a secondary figure, never value;
-  cpu_baseline: the reference's CPU construction (oracle/_ref, OpenMP on all host cores) timed
-    on a 256 MiB prefix of the same generator's text, SA and BWT built separately as the reference
-    does, MB/s = prefix bytes / (SA seconds + BWT seconds);
+  cpu_baseline: the reference's CPU construction (oracle/_ref, OpenMP on all host threads, cover
+    128) on the same whole text, SA and BWT built separately as the reference CLI does, best of 3
+    each, MB/s = n / (SA seconds + BWT seconds), with the CPU model and core count;
+  configs: secondary device-resident lines: the same text at the other cover (128: the cover the
+    configurations state), then C1 (16 MiB ACGT) and C2 (256 MiB DNA-like) at covers 128 and 32,
+    each checked on the device;
   batched: 4096 texts of log-uniform length 1 KiB - 4 MiB (even: DNA-like, odd: Markov), all
     resident in HBM, built in one batched construction, against the same texts built one at a
     time by the single-text device path; every text's outputs of both compared;
@@ -55,7 +58,36 @@ def measured_traffic(config, n):
     return None
 
 
-def cpu_baseline(text_prefix):
+def host_cpu():
+    """(model name, physical cores, logical CPUs this process may use) of the host."""
+    model, cores = "unknown", set()
+    physical = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                key, _, value = line.partition(":")
+                key, value = key.strip(), value.strip()
+                if key == "model name":
+                    model = value
+                elif key == "physical id":
+                    physical = value
+                elif key == "core id":
+                    core = value
+                elif not key and physical is not None:
+                    cores.add((physical, core))
+                    physical = core = None
+        if physical is not None:
+            cores.add((physical, core))
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return model, len(cores) or None, usable
+
+
+def cpu_baseline(text, runs=3):
+    """The reference CPU construction (oracle/_ref, OpenMP on every thread it is given, cover 128)
+    on the whole text: SA (buildSuffixArray) and BWT (buildBwt) built separately, as the reference
+    CLI's build and build --bwt do, each the best of `runs`; value = n / (best SA + best BWT)."""
     import numpy as np
 
     import oracle
@@ -63,18 +95,31 @@ def cpu_baseline(text_prefix):
         return None
     lib = oracle.ref_lib()
     threads = int(lib.ref_max_threads())
-    n = int(text_prefix.size)
+    model, physical, usable = host_cpu()
+    n = int(text.size)
     padded = np.zeros(n + oracle.TEXT_PADDING, dtype=np.uint8)
-    padded[:n] = text_prefix
+    padded[:n] = text
     sa = np.empty(n, dtype=np.int32)
-    t0 = time.perf_counter()
-    assert lib.ref_build_sa(padded.ctypes.data, n, sa.ctypes.data, 128) == 0
-    t1 = time.perf_counter()
     out = np.empty(n, dtype=np.uint8)
-    lib.ref_build_bwt(padded.ctypes.data, n, out.ctypes.data, sa.ctypes.data, 128)
-    t2 = time.perf_counter()
-    return {"value": round(n / 1e6 / (t2 - t0), 2), "unit": "MB/s", "cores": threads, "kind": "reference",
-            "sample": f"first {n >> 20} MiB of the same text; reference CPU build (cover 128), SA {t1 - t0:.2f}s + BWT {t2 - t1:.2f}s"}
+    sa_times, bwt_times = [], []
+    for r in range(runs):
+        t0 = time.perf_counter()
+        assert lib.ref_build_sa(padded.ctypes.data, n, sa.ctypes.data, 128) == 0
+        t1 = time.perf_counter()
+        primary = lib.ref_build_bwt(padded.ctypes.data, n, out.ctypes.data, sa.ctypes.data, 128)
+        t2 = time.perf_counter()
+        assert primary >= 0
+        sa_times.append(t1 - t0)
+        bwt_times.append(t2 - t1)
+        print(f"# CPU reference run {r}: SA {t1 - t0:.2f}s, BWT {t2 - t1:.2f}s", file=sys.stderr, flush=True)
+    sa_s, bwt_s = min(sa_times), min(bwt_times)
+    return {"value": round(n / 1e6 / (sa_s + bwt_s), 2), "unit": "MB/s", "cores": threads, "kind": "reference",
+            "sample": f"the whole {n >> 20} MiB text; reference CPU build (cover 128, {threads} OpenMP threads), "
+                      f"SA and BWT built separately, best of {runs} each",
+            "sa_seconds": round(sa_s, 3), "bwt_seconds": round(bwt_s, 3),
+            "sa_mb_per_s": round(n / 1e6 / sa_s, 1), "bwt_mb_per_s": round(n / 1e6 / bwt_s, 1),
+            "sa_plus_bwt_mib_per_s": round(n / 2**20 / (sa_s + bwt_s), 1),
+            "cpu_model": model, "physical_cores": physical, "usable_logical_cpus": usable}
 
 
 def cpu_texts_baseline(texts, what):
@@ -150,7 +195,8 @@ def batched_measurement(count, steps, tune=True, cpu_texts=0):
         print(f"# batched: tuned {tuned}", file=sys.stderr, flush=True)
     ctx = sadcs_amd.GpuContext(int(lengths.max()))
     batch_ctx = sadcs_amd.GpuContext(0, splitter_spacing=tuned.get("splitter_spacing", 0),
-                                     tile_splitters=tuned.get("tile_splitters", 0), batch_lengths=lengths)
+                                     tile_splitters=tuned.get("tile_splitters", 0), batch_lengths=lengths,
+                                     tile_threads=tuned.get("tile_threads", 0))
     stats = sadcs_amd.GpuStats()
 
     batch_primaries = sadcs_amd.gpu_batch_build_device(batch_ctx, d_texts.ptr, lengths, d_sa.ptr, d_bwt.ptr)  # warmup
@@ -178,7 +224,7 @@ def batched_measurement(count, steps, tune=True, cpu_texts=0):
         same = same and np.array_equal(a.to_host(np.uint8), b.to_host(np.uint8))
     for buf in (d_texts, d_seq, d_sa, d_bwt, d_sa2, d_bwt2):
         buf.free()
-    cpu = cpu_texts_baseline(texts[:cpu_texts], f"the first {cpu_texts} of the {count} texts") if cpu_texts else None
+    cpu = cpu_texts_baseline(texts[:cpu_texts], f"all {count} texts" if cpu_texts >= count else f"the first {cpu_texts} of the {count} texts") if cpu_texts else None
     return {"texts": count, "bytes": total, "mb_per_s": round(total / batch_seconds / 1e6, 1),
             "ms": round(batch_seconds * 1e3, 2), "sequential_mb_per_s": round(total / seq_seconds / 1e6, 1),
             "sequential_ms": round(seq_seconds * 1e3, 1), "speedup_vs_sequential": round(seq_seconds / batch_seconds, 2),
@@ -217,7 +263,7 @@ def served_measurement(requests, factors=(0.5, 0.8, 1.1, None), cpu=False):
         rate = factor * capacity if factor else float("inf")
         gaps = rng.exponential(1.0 / rate, len(texts)) if factor else np.zeros(len(texts))
         results = [None] * len(texts)
-        with sadcs_amd.Server(workers=2, max_batch_bytes=64 << 20) as server:
+        with sadcs_amd.Server() as server:  # the library's defaults: 3 workers, batches of at most 32 MiB
             ids = [None] * len(texts)
 
             def waiter():
@@ -239,13 +285,16 @@ def served_measurement(requests, factors=(0.5, 0.8, 1.1, None), cpu=False):
             thread.join()
             elapsed = time.perf_counter() - start
             counters = server.counters()
+            phases = server.stats()
         latency = np.array([r[3] for r in results]) * 1e3
         rates.append({"offered_x_sequential": factor if factor else "all at once",
                       "offered_requests_per_s": round(rate, 1) if factor else None,
                       "mb_per_s": round(total / elapsed / 1e6, 1), "x_sequential": round(seq_seconds / elapsed, 2),
                       "p50_ms": round(float(np.percentile(latency, 50)), 2),
                       "p99_ms": round(float(np.percentile(latency, 99)), 2),
-                      "mean_batch": round(float(np.mean([r[5] for r in results])), 2), "batches": counters["batches"]})
+                      "mean_batch": round(float(np.mean([r[5] for r in results])), 2), "batches": counters["batches"],
+                      "batch_phase_seconds": {k: round(phases[k], 4) for k in ("upload_seconds", "device_seconds",
+                                                                               "download_seconds", "copy_out_seconds")}})
         print(f"# served at {factor}x: {rates[-1]}", file=sys.stderr, flush=True)
     return {"requests": requests, "bytes": total, "sequential_mb_per_s": round(total / seq_seconds / 1e6, 1),
             "sequential_requests_per_s": round(capacity, 1), "rates": rates, "io": "host memory in and out (PCIe)",
@@ -292,6 +341,50 @@ def host_call_measurement(ctx, text, d_sa, d_bwt, primary, runs):
             "outputs_equal_device_build": same}
 
 
+def device_line(sadcs_amd, d_text, n, d_sa, d_bwt, cover, steps, warmup, label):
+    """Steps of one device-resident SA + BWT construction with the default merge configuration for
+    `cover`, outputs checked on the device afterwards: a secondary line (the configurations' stated
+    covers)."""
+    stats = sadcs_amd.GpuStats()
+    with sadcs_amd.GpuContext(n, cover=cover) as ctx:
+        for _ in range(warmup):
+            sadcs_amd.gpu_build_device(ctx, d_text.ptr, n, d_sa.ptr, d_bwt.ptr, stats)
+        sadcs_amd.gpu_synchronize()
+        phases = []
+        start = time.perf_counter()
+        for _ in range(steps):
+            primary = sadcs_amd.gpu_build_device(ctx, d_text.ptr, n, d_sa.ptr, d_bwt.ptr, stats)
+            phases.append((stats.ranking_seconds, stats.class_sort_seconds, stats.merge_seconds))
+        sadcs_amd.gpu_synchronize()
+        step = (time.perf_counter() - start) / steps
+    verdict = sadcs_amd.gpu_check_device(d_text.ptr, n, d_sa.ptr, d_bwt.ptr, primary)
+    assert verdict == (0, -1), f"{label} cover {cover}: output check failed: {verdict}"
+    ranking, classes, merge = (sum(p[i] for p in phases) / len(phases) for i in range(3))
+    line = {"workload": label, "n": n, "cover": cover, "ms_per_step": round(step * 1e3, 3),
+            "mb_per_s": round(n / step / 1e6, 1), "e2e_hbm_fraction": round(BYTES_PER_INPUT_BYTE * n / step / HBM_PEAK, 4),
+            "phases_ms": {"ranking": round(ranking * 1e3, 2), "classes": round(classes * 1e3, 2), "merge": round(merge * 1e3, 2)},
+            "character_passes": stats.character_passes, "doubling_rounds": stats.doubling_rounds,
+            "merge_kernel_ms": round(stats.merge_kernel_seconds * 1e3, 3), "output_check": "pass (device, every row)"}
+    print(f"# {label}, cover {cover}: {line}", file=sys.stderr, flush=True)
+    return line
+
+
+def config_lines(sadcs_amd, workloads, names, covers, steps=3, warmup=1):
+    """device_line for each configuration text (generated here) at each cover."""
+    lines = []
+    for name in names:
+        description, size, _, _ = workloads.CONFIGS[name]
+        text = workloads.config_text(name, padding=64)
+        d_text = sadcs_amd.DeviceBuffer.from_host(text)
+        d_sa = sadcs_amd.DeviceBuffer(4 * size)
+        d_bwt = sadcs_amd.DeviceBuffer(size)
+        for cover in covers:
+            lines.append(device_line(sadcs_amd, d_text, size, d_sa, d_bwt, cover, steps, warmup, f"{name}: {description}"))
+        for buf in (d_text, d_sa, d_bwt):
+            buf.free()
+    return lines
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,12 +393,13 @@ def main():
     ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3"])
     ap.add_argument("--cover", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample-mib", type=int, default=256)
+    ap.add_argument("--cpu-runs", type=int, default=3, help="runs of the CPU reference's SA and BWT (best of)")
     ap.add_argument("--tune-mib", type=int, default=64,
                     help="autotune the merge configuration on a prefix of this many MiB before timing (0: defaults)")
     ap.add_argument("--batch-texts", type=int, default=4096, help="texts in the batched measurement (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=300, help="requests per offered rate (0: skip)")
     ap.add_argument("--host-runs", type=int, default=3, help="host-memory calls of the same text (0: skip)")
+    ap.add_argument("--config-lines", type=int, default=1, help="secondary lines: the other cover, C1 and C2 (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device to device three times (the known-size kernel that scales HBM counters)")
     args = ap.parse_args()
@@ -352,7 +446,7 @@ def main():
         tune_seconds = time.perf_counter() - start
         print(f"# tuned on {sample >> 20} MiB in {tune_seconds:.2f}s: {tuned}", file=sys.stderr, flush=True)
     ctx = sadcs_amd.GpuContext(n, cover=args.cover, splitter_spacing=tuned.get("splitter_spacing", 0),
-                               tile_splitters=tuned.get("tile_splitters", 0))
+                               tile_splitters=tuned.get("tile_splitters", 0), tile_threads=tuned.get("tile_threads", 0))
     stats = sadcs_amd.GpuStats()
     for w in range(args.warmup):
         sadcs_amd.gpu_build_device(ctx, d_text.ptr, n, d_sa.ptr, d_bwt.ptr, stats)
@@ -388,17 +482,23 @@ def main():
     print(f"# device check of all {n} rows passed in {check_seconds:.2f}s", file=sys.stderr, flush=True)
 
     host_call = host_call_measurement(ctx, text[:n], d_sa, d_bwt, primary, args.host_runs) if args.host_runs else None
-
-    baseline = None
-    if not args.no_cpu:
-        sample = min(n, args.cpu_sample_mib << 20)
-        baseline = cpu_baseline(text[:sample])
-
     ctx.close()
+
+    # The other covers and configurations: cover 128 (the configurations' stated cover) on the same
+    # resident text, then C1 and C2 at covers 128 and 32.
+    configs = []
+    if args.config_lines:
+        other = 128 if args.cover == 32 else 32
+        configs.append(device_line(sadcs_amd, d_text, n, d_sa, d_bwt, other, 3, 1, f"{args.config}: {description}"))
+
+    baseline = None if args.no_cpu else cpu_baseline(text[:n], args.cpu_runs)
+
     for buf in (d_text, d_sa, d_bwt):
         buf.free()
+    if args.config_lines:
+        configs += config_lines(sadcs_amd, workloads, [c for c in ("c1", "c2") if c != args.config], (128, 32))
     batched = batched_measurement(args.batch_texts, 3, tune=args.tune_mib > 0,
-                                  cpu_texts=0 if args.no_cpu else min(512, args.batch_texts)) if args.batch_texts else None
+                                  cpu_texts=0 if args.no_cpu else args.batch_texts) if args.batch_texts else None
     served = served_measurement(args.serve_requests, cpu=not args.no_cpu) if args.serve_requests else None
     ranking, classes, merge = (sum(p[i] for p in phases) / len(phases) for i in range(3))
     line = {
@@ -428,6 +528,7 @@ def main():
         "output_check": {"kind": "device, every row: permutation, neighbour order, BWT and primary",
                          "result": "pass", "seconds": round(check_seconds, 3)},
         "host_call": host_call,
+        "configs": configs,
         "batched": batched,
         "served": served,
     }

@@ -66,6 +66,7 @@ typedef struct sadcs_params {
     int32_t cover;               /* 32 (default) or 128 */
     uint32_t splitter_spacing;   /* G: every G-th element of a list is a splitter */
     uint32_t tile_splitters;     /* K: sorted splitters per merge tile */
+    uint32_t tile_threads;       /* threads per merge tile: 512 (two tiles per CU) or 256 (four) */
 } sadcs_params;
 
 typedef struct sadcs_ctx sadcs_ctx;
@@ -144,9 +145,9 @@ int sadcs_hip_batch_build_device(sadcs_ctx* context, const void* d_texts, const 
                                  void* d_sa, void* d_bwt, int32_t* primaries, sadcs_stats* stats);
 
 /* ---- autotuner (no counterpart in the reference) ----
- * Builds SA + BWT of a device-resident sample with each candidate merge configuration - splitter
- * spacing G in {8, 12, 16, 24, 32} and tile splitters K at 1, 3/4 and 1/2 of the most that keep
- * two tiles per CU - and writes the fastest (by device time, best of two builds) to *best; the
+ * Builds SA + BWT of a device-resident sample with each candidate merge configuration - tile
+ * threads 512 or 256, splitter spacing G in {8, 12, 16, 24, 32} and tile splitters K at 1, 3/4 and
+ * 1/2 of the most that keep the tiles a CU runs at once in its LDS - and writes the fastest (by device time, best of two builds) to *best; the
  * cover comes from base (null: 32). count 1: d_texts is one text of lengths[0] bytes followed by
  * 64 zero bytes; count > 1: a batch in the sadcs_hip_batch_layout layout. best_seconds may be null. */
 int sadcs_hip_tune(const void* d_texts, const int64_t* lengths, int32_t count, const sadcs_params* base,
@@ -184,15 +185,17 @@ int sadcs_hip_synchronize(void);
 
 /* ---- serving: a request loop with continuous batching (no counterpart in the reference) ----
  * Submitted texts queue; whenever one of `workers` device streams is idle, a batch is formed from
- * the front of the queue (at most max_batch_bytes of text and max_batch_texts texts) and admitted
+ * the front of the queue (at most max_batch_bytes of text and max_batch_texts texts; from a deep
+ * queue, about half of the queued bytes per worker but at least 4 MiB, so that every worker has a
+ * share and outputs are copied out while other batches run) and admitted
  * if its planned device memory (about 10 bytes per input byte) fits the free part of a pool of
  * memory_budget bytes (0: 80% of the free device memory). Batches on different streams overlap.
  * A text whose plan alone exceeds the budget completes with SADCS_NOT_ENOUGH_MEMORY. */
 typedef struct sadcs_server_params {
     int64_t memory_budget;
-    int64_t max_batch_bytes;     /* 0: 256 MiB */
+    int64_t max_batch_bytes;     /* 0: 32 MiB */
     int32_t max_batch_texts;     /* 0: 4096 */
-    int32_t workers;             /* 0: 2 */
+    int32_t workers;             /* 0: 3 */
     int32_t cover;               /* 0: 32 */
     int32_t want_sa;
     int32_t want_bwt;
@@ -220,6 +223,21 @@ int sadcs_hip_server_wait(sadcs_server* server, int64_t id, int32_t* sa, uint8_t
 
 /* counters[0..4]: completed requests, batches, peak pool bytes in use, budget, most batches at once. */
 void sadcs_hip_server_counters(sadcs_server* server, int64_t* counters);
+
+/* Where the server's batches spent their time, summed over the batches so far (batches run on
+ * `workers` streams at once, so the sums can exceed the wall time). No counterpart in the reference. */
+typedef struct sadcs_server_stats {
+    int64_t batches;
+    int64_t texts;
+    int64_t bytes;               /* input bytes */
+    double upload_seconds;       /* texts to the device */
+    double device_seconds;       /* construction on the device */
+    double download_seconds;     /* outputs to the worker's host buffer after the construction */
+    double copy_out_seconds;     /* outputs from that buffer into the requests' or callers' arrays */
+    double batch_seconds;        /* the whole of each batch on its worker */
+} sadcs_server_stats;
+
+int sadcs_hip_server_stats(sadcs_server* server, sadcs_server_stats* stats);
 
 /* Finishes the queued requests, then stops. */
 void sadcs_hip_server_destroy(sadcs_server* server);

@@ -44,12 +44,20 @@ class GpuStats(ctypes.Structure):
 
 
 class GpuParams(ctypes.Structure):
-    _fields_ = [("cover", ctypes.c_int32), ("splitter_spacing", ctypes.c_uint32), ("tile_splitters", ctypes.c_uint32)]
+    _fields_ = [("cover", ctypes.c_int32), ("splitter_spacing", ctypes.c_uint32), ("tile_splitters", ctypes.c_uint32),
+                ("tile_threads", ctypes.c_uint32)]
 
 
 class ServerParams(ctypes.Structure):
     _fields_ = [("memory_budget", ctypes.c_int64), ("max_batch_bytes", ctypes.c_int64), ("max_batch_texts", ctypes.c_int32),
                 ("workers", ctypes.c_int32), ("cover", ctypes.c_int32), ("want_sa", ctypes.c_int32), ("want_bwt", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_int64), ("texts", ctypes.c_int64), ("bytes", ctypes.c_int64),
+                ("upload_seconds", ctypes.c_double), ("device_seconds", ctypes.c_double),
+                ("download_seconds", ctypes.c_double), ("copy_out_seconds", ctypes.c_double),
+                ("batch_seconds", ctypes.c_double)]
 
 
 class GpuError(RuntimeError):
@@ -102,6 +110,8 @@ def _load():
     lib.sadcs_hip_server_wait.argtypes = [P, i64, P, P, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
     lib.sadcs_hip_server_wait.restype = ctypes.c_int
+    lib.sadcs_hip_server_stats.argtypes = [P, ctypes.POINTER(ServerStats)]
+    lib.sadcs_hip_server_stats.restype = ctypes.c_int
     lib.sadcs_hip_server_counters.argtypes = [P, ctypes.POINTER(i64)]
     lib.sadcs_hip_server_counters.restype = None
     lib.sadcs_hip_server_destroy.argtypes = [P]
@@ -139,6 +149,7 @@ EXPORTED_SYMBOLS = (
     "sadcs_hip_server_create", "sadcs_hip_server_submit", "sadcs_hip_server_submit_into", "sadcs_hip_server_wait", "sadcs_hip_server_counters",
     "sadcs_hip_server_destroy", "sadcs_hip_ctx_create_batch", "sadcs_hip_device_required_bytes", "sadcs_hip_tune", "sadcs_hip_device_alloc", "sadcs_hip_device_free", "sadcs_hip_memcpy",
     "sadcs_hip_memset", "sadcs_hip_synchronize", "sadcs_hip_check_device",
+    "sadcs_hip_server_stats",
 )
 
 
@@ -180,10 +191,10 @@ def _as_bytes(text):
 class GpuContext:
     """gpusa.h:84-95: device memory for inputs up to capacity bytes, kept across calls."""
 
-    def __init__(self, capacity, cover=32, splitter_spacing=0, tile_splitters=0, batch_lengths=None):
+    def __init__(self, capacity, cover=32, splitter_spacing=0, tile_splitters=0, batch_lengths=None, tile_threads=0):
         """capacity: the longest single text; batch_lengths: plan for a batch of these lengths instead
         (sadcs_hip_ctx_create_batch)."""
-        params = GpuParams(cover, splitter_spacing, tile_splitters)
+        params = GpuParams(cover, splitter_spacing, tile_splitters, tile_threads)
         handle = ctypes.c_void_p()
         if batch_lengths is not None:
             lengths = np.ascontiguousarray(batch_lengths, dtype=np.int64)
@@ -351,7 +362,7 @@ class Server:
     """A continuous-batching construction server (sadcs_hip_server_*): submit() queues a text and
     returns an id; wait(id) blocks until its suffix array / BWT are ready."""
 
-    def __init__(self, memory_budget=0, workers=2, max_batch_bytes=0, max_batch_texts=0, cover=32,
+    def __init__(self, memory_budget=0, workers=0, max_batch_bytes=0, max_batch_texts=0, cover=32,
                  suffix_array=True, bwt=True):
         params = ServerParams(memory_budget, max_batch_bytes, max_batch_texts, workers, cover, int(suffix_array), int(bwt))
         handle = ctypes.c_void_p()
@@ -394,6 +405,12 @@ class Server:
         c = (ctypes.c_int64 * 5)()
         lib.sadcs_hip_server_counters(self.handle, c)
         return {"completed": c[0], "batches": c[1], "peak_pool_bytes": c[2], "budget": c[3], "max_concurrent_batches": c[4]}
+
+    def stats(self):
+        """Per-phase seconds summed over the batches so far (sadcs_hip_server_stats)."""
+        st = ServerStats()
+        _check(lib.sadcs_hip_server_stats(self.handle, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in ServerStats._fields_}
 
     def close(self):
         """Finishes the queued requests (they write into this object's arrays), then stops."""
@@ -481,11 +498,12 @@ def gpu_check_device(d_text, n, d_sa, d_bwt=None, primary=-1):
 def gpu_tune(d_texts, lengths, cover=32):
     """sadcs_hip_tune: the fastest merge configuration for a device-resident sample (one text with
     64 zero bytes after it, or a batch in the gpu_batch_layout layout). Returns
-    (dict(cover, splitter_spacing, tile_splitters), device seconds of the best build)."""
+    (dict(cover, splitter_spacing, tile_splitters, tile_threads), device seconds of the best build)."""
     lengths = np.ascontiguousarray(np.atleast_1d(lengths), dtype=np.int64)
-    base = GpuParams(cover, 0, 0)
+    base = GpuParams(cover, 0, 0, 0)
     best = GpuParams()
     seconds = ctypes.c_double(0)
     _check(lib.sadcs_hip_tune(ctypes.c_void_p(d_texts), lengths.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), lengths.size,
                               ctypes.byref(base), ctypes.byref(best), ctypes.byref(seconds)))
-    return {"cover": best.cover, "splitter_spacing": best.splitter_spacing, "tile_splitters": best.tile_splitters}, seconds.value
+    return {"cover": best.cover, "splitter_spacing": best.splitter_spacing, "tile_splitters": best.tile_splitters,
+            "tile_threads": best.tile_threads}, seconds.value

@@ -409,9 +409,13 @@ class ScratchStack {
 struct Params {
     int cover = 32;
     uint32_t g = 16;        // splitter spacing
-    uint32_t k = 0;         // sorted splitters per tile; 0: the most that keep two tiles per CU
+    uint32_t k = 0;         // sorted splitters per tile; 0: the most that keep the CU's tiles in its LDS
+    int threads = kMergeThreads;  // threads per merge tile (512 or 256)
 
     void apply(const sadcs_params* p) {
+        if (p && p->tile_threads) {
+            threads = static_cast<int>(p->tile_threads);
+        }
         if (p && p->cover) {
             cover = p->cover;
         }
@@ -575,7 +579,8 @@ class Pipeline {
         m_ctxMerge.segments = job.deviceSegments;
         m_ctxMerge.n = job.span;
         m_g = ctx.params.g;
-        m_k = ctx.params.k ? ctx.params.k : defaultTileSplitters<V, kBatch>(m_g);
+        m_threads = ctx.params.threads;
+        m_k = ctx.params.k ? ctx.params.k : defaultTileSplitters<V, kBatch>(m_g, m_threads);
         // Caller-owned outputs are idle until the merge writes them. The BWT buffer can hold the
         // ranks only when the merge reads records from the lines (fused SA + BWT).
         m_saScratch.reset(job.sa, job.sa ? static_cast<size_t>(job.totalLength) * sizeof(uint32_t) : 0);
@@ -636,6 +641,7 @@ class Pipeline {
     uint32_t m_samples;
     MergeCtx m_ctxMerge{};
     uint32_t m_g = 0, m_k = 0;  // splitter spacing, sorted splitters per tile
+    int m_threads = kMergeThreads;  // threads per merge tile
     ScratchStack m_saScratch;   // the caller's SA buffer before the merge
     ScratchStack m_bwtScratch;  // the caller's BWT buffer before the merge (fused builds)
 
@@ -1130,7 +1136,7 @@ class Pipeline {
     }
 
     template <int kFlags>
-    int launchTiles(const ListSet& lists, const uint32_t* rows, uint32_t tiles, const TileOut& out, uint32_t* errors, uint32_t slices = 1) {
+    int launchTiles(const ListSet& lists, RowTable rows, uint32_t tiles, const TileOut& out, uint32_t* errors, uint32_t slices = 1) {
         const uint32_t capacity = mergeCapacity<V>(m_g, m_k);
         const size_t shared = mergeSharedBytes<V, kBatch>(capacity);
         if (m_dry) {
@@ -1140,14 +1146,13 @@ class Pipeline {
         if (kTop && m_ctx.kernelTimer[0]) {
             SADCS_HIP(hipEventRecord(m_ctx.kernelTimer[0], m_stream));
         }
-        auto kernel = mergeTilesKernel<V, Texts, kFlags>;
+        auto kernel = m_threads == 256 ? mergeTilesKernel<V, Texts, kFlags, 256> : mergeTilesKernel<V, Texts, kFlags, 512>;
         SADCS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shared)));
         // Slices (a host call's top level): consecutive tile ranges, each followed by its event.
-        const uint64_t stride = static_cast<uint64_t>(lists.count) + 1;
         for (uint32_t s = 0; s < slices; s++) {
             const uint32_t first = static_cast<uint32_t>(static_cast<uint64_t>(tiles) * s / slices);
             const uint32_t last = static_cast<uint32_t>(static_cast<uint64_t>(tiles) * (s + 1) / slices);
-            kernel<<<last - first, kMergeThreads, shared, m_stream>>>(m_ctxMerge, m_texts, lists, rows + first * stride, capacity, out, errors);
+            kernel<<<last - first, m_threads, shared, m_stream>>>(m_ctxMerge, m_texts, lists, rows, first, capacity, out, errors);
             SADCS_HIP(hipGetLastError());
             if (slices > 1) {
                 SADCS_HIP(hipEventRecord(m_ctx.sliceDone[s], m_stream));
@@ -1191,7 +1196,27 @@ class Pipeline {
     }
 
     // Rows of the tiles of one level: (tiles + 1) rows of lists.count + 1 words, allocated here.
-    int levelRows(const ListSet& lists, uint32_t*& rows, uint32_t& tiles, uint32_t* errors) {
+    // The top level of a fused single-text build (cover 32) borrows them from the free tail of the
+    // caller's BWT buffer (after the ranks) when they fit the lines' spare words, where they move
+    // before the merge writes the BWT: the table then costs no memory of the library's own while
+    // every class list is live, the build's peak.
+    bool m_topRowsInBwt = false;
+
+    int allocateRows(uint32_t*& rows, uint64_t words, bool top) {
+        if (top) {
+            m_topRowsInBwt = false;
+            if (!kBatch && V == kLinePositions && m_job.wantSA && m_job.wantBWT && words <= 2 * lineCount(m_job.span) &&
+                m_bwtScratch.take(rows, words)) {
+                m_topRowsInBwt = true;
+                return SADCS_OK;
+            }
+        } else if (m_topRowsInBwt && m_bwtScratch.take(rows, words)) {
+            return SADCS_OK;  // a splitter level's table, released before the top level's tiles run
+        }
+        return m_arena.allocate(rows, words);
+    }
+
+    int levelRows(const ListSet& lists, uint32_t*& rows, uint32_t& tiles, uint32_t* errors, bool top = false) {
         const uint32_t g = m_g, k = m_k;
         const uint32_t capacity = mergeCapacity<V>(g, k);
         const int lc = lists.count;
@@ -1199,7 +1224,7 @@ class Pipeline {
 
         if (total <= capacity) {
             tiles = 1;
-            SADCS_TRY(m_arena.allocate(rows, 2 * (lc + 1)));
+            SADCS_TRY(allocateRows(rows, 2 * (lc + 1), top));
             if (!m_dry) {
                 singleTileRowsKernel<<<1, 1, 0, m_stream>>>(lists, rows);
             }
@@ -1220,7 +1245,7 @@ class Pipeline {
         const uint32_t boundaries = (count + k - 1) / k;
         tiles = boundaries;
 
-        SADCS_TRY(m_arena.allocate(rows, static_cast<uint64_t>(boundaries + 1) * (lc + 1)));
+        SADCS_TRY(allocateRows(rows, static_cast<uint64_t>(boundaries + 1) * (lc + 1), top));
         uint32_t* positions = nullptr;
         uint32_t* sorted = nullptr;
         uint32_t* counts = nullptr;
@@ -1257,7 +1282,7 @@ class Pipeline {
         unscratch(sorted);
         unscratch(positions);
         if (status != SADCS_OK) {
-            m_arena.release(rows);
+            unscratch(rows);
         }
         return status;
     }
@@ -1267,8 +1292,8 @@ class Pipeline {
         uint32_t* rows = nullptr;
         uint32_t tiles = 0;
         SADCS_TRY(levelRows(lists, rows, tiles, errors));
-        const int status = launchTiles<kFlags>(lists, rows, tiles, out, errors);
-        m_arena.release(rows);
+        const int status = launchTiles<kFlags>(lists, RowTable{rows, 0}, tiles, out, errors);
+        unscratch(rows);
         return status;
     }
 
@@ -1298,7 +1323,8 @@ class Pipeline {
         int status = SADCS_OK;
         uint32_t* rows = nullptr;
         uint32_t tiles = 0;
-        status = levelRows(lists, rows, tiles, errors);
+        status = levelRows(lists, rows, tiles, errors, true);
+        RowTable rowTable{rows, 0};
 
         // A host call downloads the top level's output rows slice by slice while later slices
         // merge (a host-side choice: the kernels and their results are those of one launch).
@@ -1309,7 +1335,7 @@ class Pipeline {
         }
         auto launchSliceRows = [&]() -> int {
             if (slices > 1 && !m_dry) {
-                sliceRowsKernel<<<1, 64, 0, m_stream>>>(rows, static_cast<uint32_t>(lists.count), tiles, slices, errors + 1);
+                sliceRowsKernel<<<1, 64, 0, m_stream>>>(rowTable, static_cast<uint32_t>(lists.count), tiles, slices, errors + 1);
                 SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords + kSliceRowWords, errors + 1, (slices + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
             }
             return SADCS_OK;
@@ -1340,6 +1366,17 @@ class Pipeline {
             if (status == SADCS_OK) {
                 status = buildLines();
             }
+            if (status == SADCS_OK && m_topRowsInBwt) {
+                // The row table leaves the BWT buffer for the lines' spare words before any BWT write.
+                const uint64_t words = static_cast<uint64_t>(tiles + 1) * (lists.count + 1);
+                if (!m_dry) {
+                    relocateRowsKernel<<<std::min<unsigned>(blocksFor(words), 4096), 256, 0, m_stream>>>(
+                        rows, words, reinterpret_cast<uint32_t*>(m_lines));
+                }
+                rowTable = RowTable{reinterpret_cast<const uint32_t*>(m_lines), 1};
+                unscratch(rows);
+                m_topRowsInBwt = false;
+            }
             if (status == SADCS_OK && m_job.wantBWT) {
                 out.bwt = m_job.bwt;
                 if (!out.bwt && m_lines && m_job.textScratch) {
@@ -1362,8 +1399,8 @@ class Pipeline {
                 status = launchSliceRows();
             }
             if (status == SADCS_OK) {
-                status = m_job.wantBWT ? launchTiles<kOutSA | kOutBWT | kInPlace>(mergeLists, rows, tiles, out, errors, slices)
-                                       : launchTiles<kOutSA | kInPlace>(mergeLists, rows, tiles, out, errors, slices);
+                status = m_job.wantBWT ? launchTiles<kOutSA | kOutBWT | kInPlace>(mergeLists, rowTable, tiles, out, errors, slices)
+                                       : launchTiles<kOutSA | kInPlace>(mergeLists, rowTable, tiles, out, errors, slices);
             }
         } else if (status == SADCS_OK && m_job.wantBWT) {
             out.bwt = m_job.bwt;
@@ -1378,7 +1415,7 @@ class Pipeline {
                 status = launchSliceRows();
             }
             if (status == SADCS_OK) {
-                status = launchTiles<kOutBWT>(mergeLists, rows, tiles, out, errors, slices);
+                status = launchTiles<kOutBWT>(mergeLists, rowTable, tiles, out, errors, slices);
             }
         }
 
@@ -1420,7 +1457,7 @@ class Pipeline {
         m_ownBWT = ownBWT;
         m_ctxMerge.lines = nullptr;
         m_arena.release(m_lines);
-        m_arena.release(rows);
+        unscratch(rows);
         if (ownFirstRows) {
             m_firstRowsOwned = firstRows;
         }
@@ -2140,11 +2177,11 @@ bool validParams(const Params& p) {
         setError("cover must be 32 or 128, not %d", p.cover);
         return false;
     }
-    if (p.g < 2 || p.g > 4096 || p.k > 4096) {
-        setError("invalid merge parameters (splitter spacing %u, tile splitters %u)", p.g, p.k);
+    if (p.g < 2 || p.g > 4096 || p.k > 4096 || (p.threads != 256 && p.threads != 512)) {
+        setError("invalid merge parameters (splitter spacing %u, tile splitters %u, tile threads %d)", p.g, p.k, p.threads);
         return false;
     }
-    const uint32_t k = p.k ? p.k : p.cover == 128 ? defaultTileSplitters<128, true>(p.g) : defaultTileSplitters<32, true>(p.g);
+    const uint32_t k = p.k ? p.k : p.cover == 128 ? defaultTileSplitters<128, true>(p.g, p.threads) : defaultTileSplitters<32, true>(p.g, p.threads);
     const uint32_t capacity = p.cover == 128 ? mergeCapacity<128>(p.g, k) : mergeCapacity<32>(p.g, k);
     const size_t shared = p.cover == 128 ? mergeSharedBytes<128, true>(capacity) : mergeSharedBytes<32, true>(capacity);
     if (capacity > 65535 || shared > 160 * 1024 - 1024) {
@@ -2243,37 +2280,39 @@ int sadcs_hip_tune(const void* d_texts, const int64_t* lengths, int32_t count, c
     std::vector<int32_t> primaries(static_cast<size_t>(count));
     double bestTime = 1e30;
     int status = SADCS_OK;
-    for (uint32_t g : {8u, 12u, 16u, 24u, 32u}) {
-        const uint32_t most = cover == 128 ? (batch ? defaultTileSplitters<128, true>(g) : defaultTileSplitters<128, false>(g))
-                                           : (batch ? defaultTileSplitters<32, true>(g) : defaultTileSplitters<32, false>(g));
-        for (uint32_t k : {most, std::max(1u, 3 * most / 4), std::max(1u, most / 2)}) {
-            sadcs_params p{};
-            p.cover = cover;
-            p.splitter_spacing = g;
-            p.tile_splitters = k;
-            sadcs_ctx* ctx = nullptr;
-            status = sadcs_hip_ctx_create_batch(lengths, count, &p, &ctx);
-            if (status != SADCS_OK) {
-                break;
-            }
-            double fastest = 1e30;
-            for (int run = 0; run < 2 && status == SADCS_OK; run++) {
-                sadcs_stats stats{};
-                status = batch ? sadcs_hip_batch_build_device(ctx, d_texts, lengths, count, d_sa, d_bwt, primaries.data(), &stats)
-                               : sadcs_hip_build_device(ctx, d_texts, static_cast<int32_t>(lengths[0]), d_sa, d_bwt, primaries.data(), &stats);
-                fastest = std::min(fastest, stats.device_seconds);
-            }
-            sadcs_hip_ctx_destroy(ctx);
-            if (status != SADCS_OK) {
-                break;
-            }
-            if (fastest < bestTime) {
-                bestTime = fastest;
-                *best = p;
-            }
+    // One candidate: a context planned for it, two builds, the faster device time kept.
+    auto tryConfig = [&](const sadcs_params& p) -> int {
+        sadcs_ctx* ctx = nullptr;
+        SADCS_TRY(sadcs_hip_ctx_create_batch(lengths, count, &p, &ctx));
+        double fastest = 1e30;
+        int s = SADCS_OK;
+        for (int run = 0; run < 2 && s == SADCS_OK; run++) {
+            sadcs_stats stats{};
+            s = batch ? sadcs_hip_batch_build_device(ctx, d_texts, lengths, count, d_sa, d_bwt, primaries.data(), &stats)
+                      : sadcs_hip_build_device(ctx, d_texts, static_cast<int32_t>(lengths[0]), d_sa, d_bwt, primaries.data(), &stats);
+            fastest = std::min(fastest, stats.device_seconds);
         }
-        if (status != SADCS_OK) {
-            break;
+        sadcs_hip_ctx_destroy(ctx);
+        if (s == SADCS_OK && fastest < bestTime) {
+            bestTime = fastest;
+            *best = p;
+        }
+        return s;
+    };
+    for (int threads : {512, 256}) {
+        for (uint32_t g : {8u, 12u, 16u, 24u, 32u}) {
+            const uint32_t most = cover == 128 ? (batch ? defaultTileSplitters<128, true>(g, threads) : defaultTileSplitters<128, false>(g, threads))
+                                               : (batch ? defaultTileSplitters<32, true>(g, threads) : defaultTileSplitters<32, false>(g, threads));
+            for (uint32_t k : {most, std::max(1u, 3 * most / 4), std::max(1u, most / 2)}) {
+                sadcs_params p{};
+                p.cover = cover;
+                p.splitter_spacing = g;
+                p.tile_splitters = k;
+                p.tile_threads = static_cast<uint32_t>(threads);
+                if (status == SADCS_OK) {
+                    status = tryConfig(p);
+                }
+            }
         }
     }
     (void)hipFree(d_bwt);
@@ -2758,8 +2797,8 @@ class Server {
         tmp.apply(&sp);
         m_params = tmp;
         m_budget = static_cast<size_t>(p.memory_budget);
-        m_workerCount = p.workers > 0 ? p.workers : 2;
-        m_maxBatchBytes = p.max_batch_bytes > 0 ? static_cast<uint64_t>(p.max_batch_bytes) : (uint64_t(256) << 20);
+        m_workerCount = p.workers > 0 ? p.workers : 3;
+        m_maxBatchBytes = p.max_batch_bytes > 0 ? static_cast<uint64_t>(p.max_batch_bytes) : (uint64_t(32) << 20);
         m_maxBatchTexts = p.max_batch_texts > 0 ? p.max_batch_texts : 4096;
         m_wantSA = p.want_sa != 0;
         m_wantBWT = p.want_bwt != 0;
@@ -2940,12 +2979,22 @@ class Server {
     std::vector<std::shared_ptr<ServerRequest>> formBatch(uint8_t*& chunk, size_t& bytes, bool& tooLarge) {
         std::vector<std::shared_ptr<ServerRequest>> batch;
         tooLarge = false;
+        // A deep queue is split so that every worker gets a share (about half of pending / workers,
+        // at least kMinBatchBytes): one worker's outputs are copied to their callers while the
+        // others' batches run on the device, instead of a few large batches running one after
+        // another with all their copies at the end.
+        uint64_t pendingBytes = 0;
+        for (auto& r : m_pending) {
+            pendingBytes += r->text.size;
+        }
+        const uint64_t share = pendingBytes / (2 * static_cast<uint64_t>(m_workerCount));
+        const uint64_t limit = std::min(m_maxBatchBytes, std::max(kMinBatchBytes, share));
         uint64_t total = 0;
         for (auto& r : m_pending) {
             if (r->text.size <= 1) {
                 continue;  // answered by the dispatcher directly
             }
-            if (!batch.empty() && (total + r->text.size > m_maxBatchBytes || static_cast<int>(batch.size()) >= m_maxBatchTexts)) {
+            if (!batch.empty() && (total + r->text.size > limit || static_cast<int>(batch.size()) >= m_maxBatchTexts)) {
                 break;
             }
             batch.push_back(r);
@@ -3139,10 +3188,12 @@ class Server {
         w.ctx->arenaBytes = w.bytes;
         w.ctx->capacity = INT32_MAX;
         sadcs_stats stats{};
+        const auto batchStart = std::chrono::steady_clock::now();
         const int status = build(*w.ctx, layout, rq, &stats);
         w.ctx->arenaBase = nullptr;  // the chunk belongs to the pool
         w.ctx->arenaBytes = 0;
         const std::string error = status == SADCS_OK ? "" : tLastError;
+        const auto copyStart = std::chrono::steady_clock::now();
         uint64_t offset = 0;
         for (size_t i = 0; i < batch.size(); i++) {
             auto& r = batch[i];
@@ -3167,12 +3218,30 @@ class Server {
             offset += n;
             r->text.allocate(0);
         }
+        std::lock_guard<std::mutex> lock(m_statsMutex);
+        m_stats.batches++;
+        m_stats.texts += static_cast<int64_t>(batch.size());
+        m_stats.bytes += static_cast<int64_t>(total);
+        m_stats.upload_seconds += stats.upload_seconds;
+        m_stats.device_seconds += stats.device_seconds;
+        m_stats.download_seconds += stats.download_seconds;
+        m_stats.copy_out_seconds += secondsSince(copyStart);
+        m_stats.batch_seconds += secondsSince(batchStart);
     }
+
+ public:
+    void statsInto(sadcs_server_stats* out) {
+        std::lock_guard<std::mutex> lock(m_statsMutex);
+        *out = m_stats;
+    }
+
+ private:
 
     Params m_params;
     size_t m_budget = 0;
-    int m_workerCount = 2;
+    int m_workerCount = 3;
     uint64_t m_maxBatchBytes = 0;
+    static constexpr uint64_t kMinBatchBytes = uint64_t(4) << 20;
     int m_maxBatchTexts = 4096;
     bool m_wantSA = true, m_wantBWT = true;
     void* m_poolBase = nullptr;
@@ -3186,6 +3255,8 @@ class Server {
     std::thread m_dispatcher;
     int64_t m_nextId = 0, m_completed = 0, m_batches = 0, m_maxConcurrent = 0;
     bool m_stopping = false, m_shutdown = false, m_stopped = false;
+    std::mutex m_statsMutex;
+    sadcs_server_stats m_stats{};
 };
 
 }  // namespace sadcs
@@ -3247,6 +3318,15 @@ int sadcs_hip_server_wait(sadcs_server* server, int64_t id, int32_t* sa, uint8_t
         return SADCS_FAILED;
     }
     return server->server->wait(id, sa, bwt, primary, latency_seconds, queued_seconds, batch_size);
+}
+
+int sadcs_hip_server_stats(sadcs_server* server, sadcs_server_stats* stats) {
+    if (!server || !stats) {
+        setError("invalid arguments");
+        return SADCS_FAILED;
+    }
+    server->server->statsInto(stats);
+    return SADCS_OK;
 }
 
 void sadcs_hip_server_counters(sadcs_server* server, int64_t* counters) {

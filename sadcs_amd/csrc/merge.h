@@ -458,13 +458,33 @@ __global__ void boundaryRowsKernel(MergeCtx c, Texts texts, ListSet lists, ListS
 
 // ---- tile layout ----------------------------------------------------------------------------
 
+// A level's tile-row table, (tiles + 1) rows of lists + 1 words: plain words, or the top level of a
+// fused single-text build, whose table moves into the lines before the merge (relocateRowsKernel):
+// word w is then the spare dword 30 + (w mod 2) of line w / 2.
+struct RowTable {
+    const uint32_t* words;
+    uint32_t inLines;
+
+    __device__ __forceinline__ uint32_t operator[](uint64_t w) const {
+        return inLines ? words[(w >> 1) * (kLineBytes / 4) + (kLineBytes / 4 - 2) + (w & 1)] : words[w];
+    }
+};
+
+// Moves a row table of `count` words into the two spare dwords of consecutive lines (count is at
+// most twice the line count; the lines' builder leaves those dwords zero for a single text).
+__global__ void relocateRowsKernel(const uint32_t* __restrict__ rows, uint64_t count, uint32_t* __restrict__ lines) {
+    for (uint64_t w = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; w < count; w += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        lines[(w >> 1) * (kLineBytes / 4) + (kLineBytes / 4 - 2) + (w & 1)] = rows[w];
+    }
+}
+
 // A tile's layout is read by wave 0 of the workgroup, one list per lane: tileRows loads the row
 // entries (a = the tile's, b = the next tile's), tilePublish checks them - every list's part must
 // lie inside the list, rows must not go backwards, the parts must add up to the difference of the
 // output bases and fit `capacity` - and writes the run starts segStart[0..lc], the list rows
 // rowStart[0..lc) (if wanted), the output base and the verdict. The caller synchronises before
 // reading them. Both are no-ops outside wave 0.
-__device__ __forceinline__ void tileRows(const ListSet& lists, const uint32_t* __restrict__ rows, uint64_t tile, uint32_t& a, uint32_t& b) {
+__device__ __forceinline__ void tileRows(const ListSet& lists, const RowTable& rows, uint64_t tile, uint32_t& a, uint32_t& b) {
     const uint32_t lane = threadIdx.x;
     const uint32_t lc = static_cast<uint32_t>(lists.count);
     a = 0;
@@ -499,7 +519,7 @@ __device__ __forceinline__ void tilePublish(const ListSet& lists, uint32_t a, ui
     }
 }
 
-__device__ __forceinline__ void tileLayout(const ListSet& lists, const uint32_t* __restrict__ rows, uint64_t tile, uint32_t capacity, uint32_t* segStart, uint32_t* rowStart, uint32_t* outBase, uint32_t* valid) {
+__device__ __forceinline__ void tileLayout(const ListSet& lists, const RowTable& rows, uint64_t tile, uint32_t capacity, uint32_t* segStart, uint32_t* rowStart, uint32_t* outBase, uint32_t* valid) {
     if (threadIdx.x < kWave) {
         uint32_t a, b;
         tileRows(lists, rows, tile, a, b);
@@ -510,7 +530,7 @@ __device__ __forceinline__ void tileLayout(const ListSet& lists, const uint32_t*
 // The first output row of every slice of a level's tiles, and the total after them: slice s starts
 // at tile s * tiles / slices, and row tile * (lists + 1) + lists of the row table is a tile's first
 // output row (tile = tiles: the total). One workgroup; slices < 64.
-__global__ void sliceRowsKernel(const uint32_t* __restrict__ rows, uint32_t lists, uint32_t tiles, uint32_t slices, uint32_t* __restrict__ out) {
+__global__ void sliceRowsKernel(RowTable rows, uint32_t lists, uint32_t tiles, uint32_t slices, uint32_t* __restrict__ out) {
     const uint32_t s = threadIdx.x;
     if (s <= slices) {
         const uint64_t tile = static_cast<uint64_t>(tiles) * s / slices;
@@ -603,8 +623,10 @@ struct TileOut {
     const uint32_t* firstSuffixRow;  // per text: local row of its suffix 0 (BWT)
 };
 
+// Threads per tile: 512 (two tiles per CU in the LDS) or 256 (four tiles per CU, each with half the
+// records), a tuned parameter. kMergeThreads is the default.
 constexpr int kMergeThreads = 512;
-constexpr int kLoadSlots = 3;  // records each thread gathers at once (default tiles hold at most 1536)
+constexpr int kLoadSlots = 3;  // records each thread gathers at once (default tiles hold at most 3 * threads)
 
 // A tile's records in LDS as a structure of arrays: a wave reading one field of 64 random records
 // then hits random banks. (As 64-byte structs, the same field of different records would map to
@@ -666,14 +688,17 @@ inline size_t mergeSharedBytes(uint32_t capacity) {
     return static_cast<size_t>(capacity) * (LdsRecords<V, kBatch>::bytesPerRecord() + 2 * sizeof(uint16_t) + 1) + 16;
 }
 
-// Tiles are latency-bound, so two workgroups per CU matter more than tile size: the default K is
-// the largest whose tiles (at most K*G + C*(G-1) records) leave room for two in the 160 KB of LDS.
-constexpr size_t kTileLdsBudget = 80 * 1024 - 1024;
+// Tiles are latency-bound, so tiles per CU matter more than tile size: the default K is the largest
+// whose tiles (at most K*G + C*(G-1) records) leave room in the 160 KB of LDS for as many tiles as
+// the CU runs at once (VGPRs allow 16 waves per CU: two 512-thread tiles or four 256-thread ones).
+inline size_t tileLdsBudget(int threads) {
+    return (threads == 256 ? 40 : 80) * 1024 - 1024;
+}
 
 template <int V, bool kBatch>
-inline uint32_t defaultTileSplitters(uint32_t g) {
+inline uint32_t defaultTileSplitters(uint32_t g, int threads = kMergeThreads) {
     const size_t perRecord = LdsRecords<V, kBatch>::bytesPerRecord() + 2 * sizeof(uint16_t) + 1;
-    const size_t records = (kTileLdsBudget - 16) / perRecord;
+    const size_t records = (tileLdsBudget(threads) - 16) / perRecord;
     const size_t spill = static_cast<size_t>(DifferenceCover<V>::kClasses) * (g - 1);
     return records > spill + g ? static_cast<uint32_t>((records - spill) / g) : 1;
 }
@@ -941,7 +966,7 @@ template <int V, class Texts, int kFlags, int kThreads>
 __device__ __forceinline__ void writeTile(const MergeCtx& c, const ListSet& lists, const LdsRecords<V, Texts::kBatch>& recs, const uint16_t* order, const uint8_t* before, const uint32_t* segStart, const uint32_t* rowStart, uint32_t outBase, uint32_t size, const TileOut& out, uint32_t* errors) {
     const int lc = lists.count;
     for (uint32_t r = threadIdx.x; r < size; r += kThreads) {
-        const uint32_t e = order[r];
+        const uint32_t e = min<uint32_t>(order[r], recs.capacity - 1);
         const uint32_t p = recs.positionOf(e);
         const uint32_t row = outBase + r;
         uint32_t start = 0, rowBase = 0, segment = 0;
@@ -985,9 +1010,9 @@ __device__ __forceinline__ void writeTile(const MergeCtx& c, const ListSet& list
 // rows of the SA buffer.
 constexpr int kPairSlots = 5;  // records per lane pair in flight: a default tile (<= 1280) in one batch
 
-template <class Texts, int kFlags>
+template <class Texts, int kFlags, int kThreads>
 __device__ __forceinline__ void loadRecordsFromLines(const MergeCtx& c, const Texts& texts, const LdsRecords<32, Texts::kBatch>& recs, const uint32_t* __restrict__ rowsIn, uint32_t size, uint8_t* before, uint16_t* indexA, uint32_t* errors) {
-    constexpr uint32_t kPairs = kMergeThreads / 2;
+    constexpr uint32_t kPairs = kThreads / 2;
     const bool textLane = (threadIdx.x & 1) == 0;
     const uint32_t cap = recs.capacity;
     for (uint32_t e0 = threadIdx.x / 2; e0 < size; e0 += kPairs * kPairSlots) {
@@ -1089,8 +1114,8 @@ __device__ __forceinline__ void loadRecordsFromLines(const MergeCtx& c, const Te
 }
 
 // One tile per workgroup: load its records into LDS, merge, write.
-template <int V, class Texts, int kFlags>
-__global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Texts texts, ListSet lists, const uint32_t* __restrict__ rows, uint32_t capacity, TileOut out, uint32_t* __restrict__ errors) {
+template <int V, class Texts, int kFlags, int kThreads>
+__global__ __launch_bounds__(kThreads) void mergeTilesKernel(MergeCtx c, Texts texts, ListSet lists, RowTable rows, uint32_t firstTile, uint32_t capacity, TileOut out, uint32_t* __restrict__ errors) {
     using Record = MergeRecord<V, Texts::kBatch>;
     extern __shared__ __align__(16) unsigned char shared[];
     LdsRecords<V, Texts::kBatch> recs;
@@ -1105,7 +1130,7 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
     __shared__ const uint32_t* listPtr[kMaxLists];  // indexed per lane: LDS, not kernel-argument loads
 
     const int lc = lists.count;
-    const uint64_t tile = blockIdx.x;
+    const uint64_t tile = static_cast<uint64_t>(firstTile) + blockIdx.x;
     const TieTables tables = setupTables<V>();
 
     if (threadIdx.x < static_cast<uint32_t>(lc)) {
@@ -1128,19 +1153,19 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
     if constexpr (V == kLinePositions && (kFlags & kInPlace) != 0) {
         fromLines = c.lines != nullptr;
         if (fromLines) {
-            loadRecordsFromLines<Texts, kFlags>(c, texts, recs, out.sa + outBase, size, before, indexA, errors);
+            loadRecordsFromLines<Texts, kFlags, kThreads>(c, texts, recs, out.sa + outBase, size, before, indexA, errors);
         }
     }
     // Otherwise kLoadSlots records per thread at a time: every slot's position first, then every
-    // slot's record gathers, so a tile of up to kLoadSlots * kMergeThreads records costs two dependent
+    // slot's record gathers, so a tile of up to kLoadSlots * kThreads records costs two dependent
     // round trips (straight-line code, clamped indices); the BWT character before each record too.
     if (!fromLines) {
-        for (uint32_t e0 = threadIdx.x; e0 < size; e0 += kLoadSlots * kMergeThreads) {
+        for (uint32_t e0 = threadIdx.x; e0 < size; e0 += kLoadSlots * kThreads) {
             uint32_t pos[kLoadSlots];
             int l = 0;
 #pragma unroll
             for (int h = 0; h < kLoadSlots; h++) {
-                const uint32_t e = min(e0 + h * kMergeThreads, size - 1);
+                const uint32_t e = min(e0 + h * kThreads, size - 1);
                 if constexpr ((kFlags & kInPlace) != 0) {
                     pos[h] = out.sa[outBase + e];
                 } else {
@@ -1154,7 +1179,7 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
 #pragma unroll
             for (int h = 0; h < kLoadSlots; h++) {
                 if (pos[h] >= c.n) {
-                    if (e0 + h * kMergeThreads < size) {
+                    if (e0 + h * kThreads < size) {
                         atomicOr(errors, 2u);
                     }
                     pos[h] = 0;
@@ -1163,7 +1188,7 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
             }
 #pragma unroll
             for (int h = 0; h < kLoadSlots; h++) {
-                const uint32_t e = e0 + h * kMergeThreads;
+                const uint32_t e = e0 + h * kThreads;
                 if (e < size) {
                     Record r;
                     uint8_t characterBefore;
@@ -1179,8 +1204,8 @@ __global__ __launch_bounds__(kMergeThreads) void mergeTilesKernel(MergeCtx c, Te
     }
     __syncthreads();
 
-    const uint16_t* order = mergeRuns<V, Texts, kMergeThreads>(c, tables, recs, indexA, indexB, bounds, static_cast<uint32_t>(lc), size, true, [] {});
-    writeTile<V, Texts, kFlags, kMergeThreads>(c, lists, recs, order, before, segStart, rowStart, outBase, size, out, errors);
+    const uint16_t* order = mergeRuns<V, Texts, kThreads>(c, tables, recs, indexA, indexB, bounds, static_cast<uint32_t>(lc), size, true, [] {});
+    writeTile<V, Texts, kFlags, kThreads>(c, lists, recs, order, before, segStart, rowStart, outBase, size, out, errors);
 }
 
 // Per text: the row of its first suffix (the number of its suffixes smaller than it), and the

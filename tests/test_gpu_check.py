@@ -160,5 +160,7 @@ def test_largest_text_passes_the_device_check(gpu):
         stats = gpu.GpuStats()
         primary = gpu.gpu_build_device(ctx, d_text.ptr, n, d_sa.ptr, d_bwt.ptr, stats)
     print(f"n={n}: device {stats.device_seconds * 1e3:.1f} ms, peak {stats.device_bytes / n:.2f} B/byte", flush=True)
-    assert stats.device_bytes <= 10.1 * n
+    # The 10n budget: text, SA and BWT (6n, the caller's) plus the library's own 4n (the class lists
+    # at the peak), and a few KiB of fixed tables (text padding, segment table, error flags).
+    assert stats.device_bytes <= 10 * n + (64 << 10)
     assert gpu.gpu_check_device(d_text.ptr, n, d_sa.ptr, d_bwt.ptr, primary) == (0, -1)

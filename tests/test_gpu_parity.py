@@ -219,9 +219,9 @@ def test_merge_parameters(gpu):
     """Small splitter spacing and tiles force several merge levels on modest inputs."""
     rng = oracle.Rng(9)
     text = rng.random_text(300_000, 3)
-    for g, k in [(2, 1), (4, 3), (16, 2), (128, 4)]:
-        with gpu.GpuContext(len(text), splitter_spacing=g, tile_splitters=k) as ctx:
-            check_case(gpu, f"merge g={g} k={k}", text, ctx)
+    for g, k, threads in [(2, 1, 512), (4, 3, 512), (16, 2, 512), (128, 4, 512), (8, 0, 256), (16, 3, 256)]:
+        with gpu.GpuContext(len(text), splitter_spacing=g, tile_splitters=k, tile_threads=threads) as ctx:
+            check_case(gpu, f"merge g={g} k={k} threads={threads}", text, ctx)
 
 
 def test_batch_equals_single(gpu):
@@ -341,8 +341,10 @@ def test_tuner_picks_a_working_configuration(gpu):
     d_text = gpu.DeviceBuffer.from_host(text, extra=64)
     best, seconds = gpu.gpu_tune(d_text.ptr, [text.size])
     assert best["cover"] == 32 and best["splitter_spacing"] in (8, 12, 16, 24, 32) and best["tile_splitters"] >= 1
+    assert best["tile_threads"] in (256, 512)
     assert seconds > 0
-    with gpu.GpuContext(text.size, splitter_spacing=best["splitter_spacing"], tile_splitters=best["tile_splitters"]) as ctx:
+    with gpu.GpuContext(text.size, splitter_spacing=best["splitter_spacing"], tile_splitters=best["tile_splitters"],
+                        tile_threads=best["tile_threads"]) as ctx:
         tuned = gpu.gpu_build(text, True, True, context=ctx)
     plain = gpu.gpu_build(text, True, True)
     assert np.array_equal(tuned[0], plain[0]) and tuned[1:] == plain[1:]
