@@ -173,6 +173,16 @@ enum {
 int sadcs_hip_check_device(const void* d_text, int32_t n, const void* d_sa, const void* d_bwt, int32_t primary,
                            int64_t* result);
 
+/* sadcs_hip_check_device without any allocation: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The same checks and results, for n up to 8388608 (2^23) entries; larger n
+ * gives SADCS_NOT_ENOUGH_MEMORY and checks nothing. The inverse lives in a device array of the
+ * library (32 MiB + 256 B, held by every device the library runs on until the process ends),
+ * shared by every call on that device: calls that overlap on one device race and must not be
+ * made. Each call clears what it uses; nothing carries over from the last call. */
+int sadcs_hip_check_device_noalloc(const void* d_text, int32_t n, const void* d_sa, const void* d_bwt, int32_t primary,
+                                   int64_t* result);
+
 /* ---- device memory for callers without their own HIP setup (no counterpart in the reference) ----
  * The library's HIP runtime allocates and copies, so a caller that keeps texts and outputs on the
  * device needs no second runtime in the process. kind: 1 host to device, 2 device to host,

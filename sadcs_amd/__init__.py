@@ -135,6 +135,8 @@ def _load():
     lib.sadcs_hip_synchronize.restype = ctypes.c_int
     lib.sadcs_hip_check_device.argtypes = [P, i32, P, P, i32, ctypes.POINTER(i64)]
     lib.sadcs_hip_check_device.restype = ctypes.c_int
+    lib.sadcs_hip_check_device_noalloc.argtypes = [P, i32, P, P, i32, ctypes.POINTER(i64)]
+    lib.sadcs_hip_check_device_noalloc.restype = ctypes.c_int
     return lib
 
 
@@ -148,7 +150,7 @@ EXPORTED_SYMBOLS = (
     "sadcs_hip_batch_required_bytes", "sadcs_hip_batch_layout", "sadcs_hip_batch_build_device",
     "sadcs_hip_server_create", "sadcs_hip_server_submit", "sadcs_hip_server_submit_into", "sadcs_hip_server_wait", "sadcs_hip_server_counters",
     "sadcs_hip_server_destroy", "sadcs_hip_ctx_create_batch", "sadcs_hip_device_required_bytes", "sadcs_hip_tune", "sadcs_hip_device_alloc", "sadcs_hip_device_free", "sadcs_hip_memcpy",
-    "sadcs_hip_memset", "sadcs_hip_synchronize", "sadcs_hip_check_device",
+    "sadcs_hip_memset", "sadcs_hip_synchronize", "sadcs_hip_check_device", "sadcs_hip_check_device_noalloc",
     "sadcs_hip_server_stats",
 )
 

@@ -2675,32 +2675,17 @@ int sadcs_hip_batch_build_device(sadcs_ctx* context, const void* d_texts, const 
     return status;
 }
 
-int sadcs_hip_check_device(const void* d_text, int32_t n, const void* d_sa, const void* d_bwt, int32_t primary, int64_t* result) {
-    tLastError[0] = '\0';
-    if (!result) {
-        setError("invalid arguments");
-        return SADCS_FAILED;
-    }
-    result[0] = SADCS_CHECK_PASS;
-    result[1] = -1;
-    if (n <= 0) {
-        return SADCS_OK;
-    }
-    if (!d_text || !d_sa || (reinterpret_cast<uintptr_t>(d_sa) & 3) != 0) {
-        setError("the device text and a 4-byte aligned device suffix array are needed");
-        return SADCS_FAILED;
-    }
-    // inverse: n int32, then the first failing index of each check.
-    const size_t inverseBytes = (static_cast<size_t>(n) * sizeof(int32_t) + 255) / 256 * 256;
-    void* memory = nullptr;
-    SADCS_HIP(hipMalloc(&memory, inverseBytes + 256));
-    std::unique_ptr<void, void (*)(void*)> owner(memory, [](void* m) { (void)hipFree(m); });
-    auto* inverse = static_cast<int32_t*>(memory);
-    auto* first = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(memory) + inverseBytes);
+namespace {
+// The device check's three passes with a caller-provided inverse (n words) and result words
+// (kCheckCount), on the null stream, then the verdict. The inverse and result words are cleared
+// here with a fill kernel.
+int runDeviceCheck(const void* d_text, int32_t n, const void* d_sa, const void* d_bwt, int32_t primary,
+                   int32_t* inverse, uint32_t* first, int64_t* result) {
     const auto* text = static_cast<const uint8_t*>(d_text);
     const auto* sa = static_cast<const int32_t*>(d_sa);
     const uint32_t blocks = std::min<uint32_t>(kCheckBlocks, (static_cast<uint32_t>(n) + kCheckThreads - 1) / kCheckThreads);
-    SADCS_HIP(hipMemset(memory, 0xff, inverseBytes + kCheckCount * sizeof(uint32_t)));
+    fillWordsKernel<<<blocks, kCheckThreads>>>(reinterpret_cast<uint32_t*>(inverse), static_cast<uint64_t>(n), ~0u);
+    fillWordsKernel<<<1, kCheckThreads>>>(first, kCheckCount, ~0u);
     checkRangeKernel<<<blocks, kCheckThreads>>>(sa, static_cast<uint32_t>(n), inverse, first);
     checkOrderKernel<<<blocks, kCheckThreads>>>(text, sa, static_cast<uint32_t>(n), inverse, first);
     if (d_bwt) {
@@ -2727,6 +2712,89 @@ int sadcs_hip_check_device(const void* d_text, int32_t n, const void* d_sa, cons
         }
     }
     return SADCS_OK;
+}
+
+// Argument checks shared by both entry points; returns -1 to go on, else the status to return.
+int checkArguments(const void* d_text, int32_t n, const void* d_sa, int64_t* result) {
+    tLastError[0] = '\0';
+    if (!result) {
+        setError("invalid arguments");
+        return SADCS_FAILED;
+    }
+    result[0] = SADCS_CHECK_PASS;
+    result[1] = -1;
+    if (n <= 0) {
+        return SADCS_OK;
+    }
+    if (!d_text || !d_sa || (reinterpret_cast<uintptr_t>(d_sa) & 3) != 0) {
+        setError("the device text and a 4-byte aligned device suffix array are needed");
+        return SADCS_FAILED;
+    }
+    return -1;
+}
+
+// The allocation-free check's scratch: the inverse of up to kNoAllocCheckMax entries, then the
+// result words. A file-scope device array: every device the library runs on holds its own copy
+// (32 MiB + 256 B) from module load until the process ends. Host code never passes the array's
+// name to a kernel (in host code it is a stand-in, not a device address): each device's address
+// comes from hipGetSymbolAddress, taken once per device with that device current.
+constexpr uint32_t kNoAllocCheckMax = 1u << 23;
+constexpr uint32_t kNoAllocScratchWords = kNoAllocCheckMax + 64;
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+uint32_t* gScratchAddress[kMaxDevices] = {};
+
+// This device's address of `scratch`, looked up on the first call with the device current.
+int noAllocScratch(uint32_t*& out) {
+    int device = 0;
+    SADCS_HIP(hipGetDevice(&device));
+    if (device < 0 || device >= kMaxDevices) {
+        setError("device %d is beyond the %d devices the allocation-free check tracks", device, kMaxDevices);
+        return SADCS_FAILED;
+    }
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratchAddress[device]) {
+        void* address = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&address, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !address) {
+            setError("the allocation-free check's scratch on device %d: %s", device, hipGetErrorString(e));
+            return SADCS_FAILED;
+        }
+        gScratchAddress[device] = static_cast<uint32_t*>(address);
+    }
+    out = gScratchAddress[device];
+    return SADCS_OK;
+}
+}  // namespace
+
+int sadcs_hip_check_device(const void* d_text, int32_t n, const void* d_sa, const void* d_bwt, int32_t primary, int64_t* result) {
+    const int early = checkArguments(d_text, n, d_sa, result);
+    if (early >= 0) {
+        return early;
+    }
+    // inverse: n int32, then the first failing index of each check.
+    const size_t inverseBytes = (static_cast<size_t>(n) * sizeof(int32_t) + 255) / 256 * 256;
+    void* memory = nullptr;
+    SADCS_HIP(hipMalloc(&memory, inverseBytes + 256));
+    std::unique_ptr<void, void (*)(void*)> owner(memory, [](void* m) { (void)hipFree(m); });
+    return runDeviceCheck(d_text, n, d_sa, d_bwt, primary, static_cast<int32_t*>(memory),
+                          reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(memory) + inverseBytes), result);
+}
+
+int sadcs_hip_check_device_noalloc(const void* d_text, int32_t n, const void* d_sa, const void* d_bwt, int32_t primary, int64_t* result) {
+    const int early = checkArguments(d_text, n, d_sa, result);
+    if (early >= 0) {
+        return early;
+    }
+    if (static_cast<uint32_t>(n) > kNoAllocCheckMax) {
+        setError("the allocation-free check takes at most %u entries (got %d); use sadcs_hip_check_device", kNoAllocCheckMax, n);
+        return SADCS_NOT_ENOUGH_MEMORY;
+    }
+    uint32_t* base = nullptr;
+    SADCS_TRY(noAllocScratch(base));
+    return runDeviceCheck(d_text, n, d_sa, d_bwt, primary, reinterpret_cast<int32_t*>(base), base + kNoAllocCheckMax, result);
 }
 
 int64_t sadcs_hip_batch_required_bytes(const int64_t* lengths, int32_t count) {
