@@ -29,7 +29,8 @@ Printed as one JSON line, with:
   served: requests (lognormal lengths, median 256 KiB, sigma 1.5, capped at 64 MiB; 45% ACGT,
     45% protein-like, 10% periodic) arriving as a Poisson stream at 0.5, 0.8 and 1.1 times the
     measured one-at-a-time capacity, through the continuous-batching server (host memory in and
-    out); p50/p99 latency from submission to result and MB/s over the run.
+    out); p50/p99 latency from submission to result and MB/s over the run. Further streams at 2, 3
+    and 4 times that capacity and every request at once show where the server stops keeping up.
 """
 import argparse
 import json
@@ -232,7 +233,7 @@ def batched_measurement(count, steps, tune=True, cpu_texts=0):
             "merge_params": tuned or "defaults", "cpu_baseline": cpu}
 
 
-def served_measurement(requests, factors=(0.5, 0.8, 1.1, None), cpu=False):
+def served_measurement(requests, factors=(0.5, 0.8, 1.1, 2.0, 3.0, 4.0, None), cpu=False):
     """Poisson request streams through the serving loop, against one-at-a-time host builds. A factor
     of None submits every request at once: the server's capacity with full batches."""
     import threading
