@@ -859,6 +859,37 @@ class Pipeline {
         const int rankBits = bitWidth(M);
 
         while (u > 0) {
+            if (!first && u <= kTailMax) {
+                // The remaining passes in one workgroup, with one wait for all of them.
+                TailParams tp{};
+                tp.rp = rp;
+                tp.codes = codes;
+                tp.charBits = sigma == 256 ? 9 : charBits;
+                tp.rankBits = rankBits;
+                tp.maxLength = m_job.maxLength;
+                tp.depth = depth;
+                tp.listT = values[listBuffer];
+                tp.listOrdinal = reinterpret_cast<const uint32_t*>(keys[listBuffer]);
+                tp.groupIds = groupIds[gcur];
+                tp.u = u;
+                tp.groups = groups;
+                tp.status = counts;
+                const size_t tailShared = rankTailSharedBytes();
+                SADCS_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(rankTailKernel<V, Texts>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              static_cast<int>(tailShared)));
+                rankTailKernel<V, Texts><<<1, kTailThreads, tailShared, m_stream>>>(tp, m_texts);
+                SADCS_HIP(hipGetLastError());
+                SADCS_HIP(hipMemcpyAsync(m_ctx.hostWords, counts, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, m_stream));
+                SADCS_HIP(hipStreamSynchronize(m_stream));
+                times.characterPasses += static_cast<int>(m_ctx.hostWords[0]);
+                times.doublingRounds += static_cast<int>(m_ctx.hostWords[1]);
+                if (m_ctx.hostWords[2] != 0) {
+                    setError("sample ranking: groups left after doubling past the longest text (internal error)");
+                    cleanup();
+                    return SADCS_FAILED;
+                }
+                break;
+            }
             const int ordinalBits = groups > 1 ? bitWidth(groups - 1) : 0;
             PassSpec spec{};
             if (depth < static_cast<uint64_t>(V)) {

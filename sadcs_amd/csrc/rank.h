@@ -493,6 +493,233 @@ __global__ __launch_bounds__(256) void scatterResolvedRanksKernel(const uint32_t
     }
 }
 
+// ---- the last passes in one workgroup ----------------------------------------------------------
+//
+// Once few suffixes are unresolved, a pass costs a dozen small launches and a host round trip to
+// learn the next list's size. rankTailKernel runs every remaining pass in one workgroup instead:
+// the list, the group ids, the keys and the scans live in LDS, the workgroup loops until the list
+// is empty, and the host waits once. A pass is the same as above - the keys of makePassKeysKernel,
+// a sort by key, the mark scan of MarkConsumer - so ids, ranks and the sorted sample come out the
+// same. The sort is a bitonic network on (key, list index); ties among equal keys only reorder
+// members of one unresolved group, whose id they share.
+constexpr uint32_t kTailMax = 2048;  // list entries the workgroup takes
+constexpr int kTailThreads = 1024;
+constexpr int kTailItems = kTailMax / kTailThreads;  // consecutive sorted entries per thread in the scan
+
+struct TailParams {
+    RankParams rp;
+    CodeTable codes;
+    int charBits;         // bits per character code; 9: all 256 bytes occur (code = byte + 1)
+    int rankBits;         // bits of a doubling subkey (group id + 1)
+    uint32_t maxLength;   // longest text: a doubling step as long is an internal error
+    uint64_t depth;       // characters every group shares
+    const uint32_t* listT;
+    const uint32_t* listOrdinal;
+    const uint32_t* groupIds;
+    uint32_t u, groups;
+    uint32_t* status;     // out: character passes, doubling rounds, error (1: doubling beyond maxLength)
+};
+
+inline size_t rankTailSharedBytes() {
+    return static_cast<size_t>(kTailMax) * (sizeof(uint64_t) + 6 * sizeof(uint32_t) + sizeof(uint16_t));
+}
+
+template <int V, class Texts>
+__global__ __launch_bounds__(kTailThreads) void rankTailKernel(TailParams tp, Texts texts) {
+    extern __shared__ __align__(16) unsigned char shared[];
+    uint64_t* key = reinterpret_cast<uint64_t*>(shared);
+    uint32_t* listT[2] = {reinterpret_cast<uint32_t*>(key + kTailMax), nullptr};
+    listT[1] = listT[0] + kTailMax;
+    uint32_t* listOrdinal[2] = {listT[1] + kTailMax, listT[1] + 2 * kTailMax};
+    uint32_t* groupId[2] = {listOrdinal[1] + kTailMax, listOrdinal[1] + 2 * kTailMax};
+    uint16_t* index = reinterpret_cast<uint16_t*>(groupId[1] + kTailMax);
+    __shared__ uint8_t code[256];
+    __shared__ MarkVec waveTotal[kTailThreads / kWave];
+
+    const uint32_t tid = threadIdx.x;
+    const RankParams& rp = tp.rp;
+    for (uint32_t x = tid; x < 256; x += kTailThreads) {
+        code[x] = tp.codes.code[x];
+    }
+    uint32_t u = min(tp.u, kTailMax), groups = min(tp.groups, kTailMax);
+    for (uint32_t i = tid; i < u; i += kTailThreads) {
+        listT[0][i] = tp.listT[i];
+        listOrdinal[0][i] = tp.listOrdinal[i];
+    }
+    for (uint32_t g = tid; g < groups; g += kTailThreads) {
+        groupId[0][g] = tp.groupIds[g];
+    }
+    __syncthreads();
+
+    uint64_t depth = tp.depth;
+    uint32_t characterPasses = 0, doublingRounds = 0, error = 0;
+    int cur = 0;
+    while (u > 0) {
+        // The pass, as the host would choose it.
+        const int ordinalBits = groups > 1 ? 32 - __clz(groups - 1) : 0;
+        bool doubling;
+        uint32_t offset;
+        int chars = 0, subkeyBits;
+        if (depth < static_cast<uint64_t>(V)) {
+            doubling = false;
+            offset = static_cast<uint32_t>(depth);
+            chars = max(1, (64 - ordinalBits) / tp.charBits);
+            subkeyBits = chars * tp.charBits;
+        } else {
+            const uint64_t h = depth / V * V;
+            if (h >= tp.maxLength) {
+                error = 1;
+                break;
+            }
+            doubling = true;
+            offset = static_cast<uint32_t>(h);
+            subkeyBits = tp.rankBits;
+        }
+
+        // Keys (makePassKeysKernel after the first pass); entries past u sort last.
+        uint32_t width = 1;
+        while (width < u) {
+            width <<= 1;
+        }
+        for (uint32_t i = tid; i < width; i += kTailThreads) {
+            if (i < u) {
+                const uint32_t t = listT[cur][i];
+                const uint32_t p = unpackIndex<V>(t);
+                const uint32_t end = texts.endOf(p);
+                const uint32_t q = p + offset;
+                uint64_t subkey = 0;
+                if (doubling) {
+                    subkey = q < end ? uint64_t(rp.ranks[packPosition<V>(q)]) + 1 : 0;
+                } else {
+                    const uint32_t remaining = q < end ? end - q : 0;
+                    uint64_t w = 0;
+                    for (int c = 0; c < chars; c++) {
+                        if ((c & 7) == 0) {
+                            w = static_cast<uint32_t>(c) < remaining ? load8(rp.text, q + c) : 0;
+                        }
+                        const uint32_t byte = static_cast<uint32_t>(w >> 56);
+                        w <<= 8;
+                        const uint32_t ch = static_cast<uint32_t>(c) < remaining ? (tp.charBits == 9 ? byte + 1 : code[byte]) : 0;
+                        subkey = (subkey << tp.charBits) | ch;
+                    }
+                }
+                key[i] = (subkeyBits < 64 ? (uint64_t(listOrdinal[cur][i]) << subkeyBits) : 0) | subkey;
+                index[i] = static_cast<uint16_t>(i);
+            } else {
+                key[i] = ~uint64_t(0);
+                index[i] = 0xFFFF;
+            }
+        }
+        __syncthreads();
+
+        // Bitonic sort of (key, index), ascending.
+        for (uint32_t k = 2; k <= width; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = tid; i < width; i += kTailThreads) {
+                    const uint32_t l = i ^ j;
+                    if (l > i) {
+                        const uint64_t a = key[i], b = key[l];
+                        const uint16_t ia = index[i], ib = index[l];
+                        const bool greater = a > b || (a == b && ia > ib);
+                        if (greater == ((i & k) == 0)) {
+                            key[i] = b;
+                            key[l] = a;
+                            index[i] = ib;
+                            index[l] = ia;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+        // The mark scan (MarkProducer/MarkConsumer) over sorted entries tid * kTailItems + 0..3.
+        const int shift = subkeyBits < 64 ? subkeyBits : 0;
+        const uint64_t groupMask = subkeyBits < 64 ? ~uint64_t(0) : 0;
+        MarkVec item[kTailItems];
+        MarkVec running = MarkVec::zero();
+#pragma unroll
+        for (int j = 0; j < kTailItems; j++) {
+            const uint32_t i = tid * kTailItems + j;
+            MarkVec r = MarkVec::zero();
+            if (i < u) {
+                const uint64_t k0 = key[i];
+                const uint64_t previous = key[i > 0 ? i - 1 : 0];
+                const uint64_t next = key[i + 1 < u ? i + 1 : i];
+                const bool first = i == 0, last = i + 1 == u;
+                const bool head = first | (previous != k0);
+                const bool groupStart = first | ((((previous >> shift) ^ (k0 >> shift)) & groupMask) != 0);
+                const bool resolved = head & (last | (next != k0));
+                r.v[0] = head ? i : 0;
+                r.v[1] = groupStart ? i : 0;
+                r.v[2] = resolved ? 0 : 1;
+                r.v[3] = (!resolved && head) ? 1 : 0;
+            }
+            item[j] = r;
+            running = MarkVec::combine(running, r);
+        }
+        // Exclusive prefix of this thread's total over the workgroup.
+        const MarkVec inclusive = MarkVec::waveInclusive(running);
+        const uint32_t lane = tid % kWave, wave = tid / kWave;
+        if (lane == kWave - 1) {
+            waveTotal[wave] = inclusive;
+        }
+        __syncthreads();
+        MarkVec before = MarkVec::shiftUp(inclusive);
+        MarkVec all = MarkVec::zero();
+        for (uint32_t w = 0; w < kTailThreads / kWave; w++) {
+            if (w < wave) {
+                before = MarkVec::combine(waveTotal[w], before);
+            }
+            all = MarkVec::combine(all, waveTotal[w]);
+        }
+        const int next = 1 - cur;
+#pragma unroll
+        for (int j = 0; j < kTailItems; j++) {
+            const uint32_t i = tid * kTailItems + j;
+            const MarkVec it = item[j];
+            if (i < u) {
+                const uint64_t k0 = key[i];
+                const uint32_t ordinal = subkeyBits >= 64 ? 0u : static_cast<uint32_t>(k0 >> subkeyBits);
+                const uint32_t lastHead = max(before.v[0], it.v[0]);
+                const uint32_t groupStart = max(before.v[1], it.v[1]);
+                const uint32_t id = groupId[cur][ordinal] + (lastHead - groupStart);
+                const uint32_t t = listT[cur][index[i]];
+                rp.ranks[t] = id;
+                if (it.v[2] == 0) {
+                    if (id < rp.samples) {
+                        rp.sortedSample[id] = unpackIndex<V>(t);
+                    }
+                } else {
+                    const uint32_t slot = before.v[2];
+                    listT[next][slot] = t;
+                    listOrdinal[next][slot] = before.v[3] + it.v[3] - 1;
+                    if (it.v[3]) {
+                        groupId[next][before.v[3]] = id;
+                    }
+                }
+            }
+            before = MarkVec::combine(before, it);
+        }
+        u = all.v[2];
+        groups = all.v[3];
+        cur = next;
+        if (doubling) {
+            depth += offset;
+            doublingRounds++;
+        } else {
+            depth += static_cast<uint64_t>(chars);
+            characterPasses++;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        tp.status[0] = characterPasses;
+        tp.status[1] = doublingRounds;
+        tp.status[2] = error;
+    }
+}
+
 __global__ void initGroupIdsKernel(const Segment* __restrict__ segments, uint32_t count, uint32_t* __restrict__ groupIds) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < count) {
