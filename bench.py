@@ -233,7 +233,7 @@ def batched_measurement(count, steps, tune=True, cpu_texts=0):
             "merge_params": tuned or "defaults", "cpu_baseline": cpu}
 
 
-def served_measurement(requests, factors=(0.5, 0.8, 1.1, 2.0, 3.0, 4.0, None), cpu=False):
+def served_measurement(requests, factors=(0.5, 0.8, 1.1, 2.0, 3.0, 4.0, None), cpu=False, workers=0, batch_mib=0):
     """Poisson request streams through the serving loop, against one-at-a-time host builds. A factor
     of None submits every request at once: the server's capacity with full batches."""
     import threading
@@ -264,7 +264,8 @@ def served_measurement(requests, factors=(0.5, 0.8, 1.1, 2.0, 3.0, 4.0, None), c
         rate = factor * capacity if factor else float("inf")
         gaps = rng.exponential(1.0 / rate, len(texts)) if factor else np.zeros(len(texts))
         results = [None] * len(texts)
-        with sadcs_amd.Server() as server:  # the library's defaults: 3 workers, batches of at most 32 MiB
+        # 0: the library's defaults (5 workers, batches of at most 32 MiB)
+        with sadcs_amd.Server(workers=workers, max_batch_bytes=batch_mib << 20) as server:
             ids = [None] * len(texts)
 
             def waiter():
@@ -399,6 +400,8 @@ def main():
                     help="autotune the merge configuration on a prefix of this many MiB before timing (0: defaults)")
     ap.add_argument("--batch-texts", type=int, default=4096, help="texts in the batched measurement (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=300, help="requests per offered rate (0: skip)")
+    ap.add_argument("--serve-workers", type=int, default=0, help="server workers (0: the library default)")
+    ap.add_argument("--serve-batch-mib", type=int, default=0, help="largest server batch in MiB (0: the library default)")
     ap.add_argument("--host-runs", type=int, default=3, help="host-memory calls of the same text (0: skip)")
     ap.add_argument("--config-lines", type=int, default=1, help="secondary lines: the other cover, C1 and C2 (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -500,7 +503,8 @@ def main():
         configs += config_lines(sadcs_amd, workloads, [c for c in ("c1", "c2") if c != args.config], (128, 32))
     batched = batched_measurement(args.batch_texts, 3, tune=args.tune_mib > 0,
                                   cpu_texts=0 if args.no_cpu else args.batch_texts) if args.batch_texts else None
-    served = served_measurement(args.serve_requests, cpu=not args.no_cpu) if args.serve_requests else None
+    served = served_measurement(args.serve_requests, cpu=not args.no_cpu, workers=args.serve_workers,
+                                batch_mib=args.serve_batch_mib) if args.serve_requests else None
     ranking, classes, merge = (sum(p[i] for p in phases) / len(phases) for i in range(3))
     line = {
         "metric": METRIC,

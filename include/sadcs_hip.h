@@ -205,7 +205,7 @@ typedef struct sadcs_server_params {
     int64_t memory_budget;
     int64_t max_batch_bytes;     /* 0: 32 MiB */
     int32_t max_batch_texts;     /* 0: 4096 */
-    int32_t workers;             /* 0: 3 */
+    int32_t workers;             /* 0: 5 */
     int32_t cover;               /* 0: 32 */
     int32_t want_sa;
     int32_t want_bwt;

@@ -1710,9 +1710,28 @@ public:
         return pool;
     }
 
+    struct Piece {
+        void* dst;
+        const void* src;
+        size_t bytes;
+    };
+
     void copy(void* dst, const void* src, size_t bytes) {
+        const Piece piece{dst, src, bytes};
+        copyPieces(&piece, 1);
+    }
+
+    // Copies every piece, the whole list split evenly over the pool by bytes (pieces are cut where
+    // the slices fall), so many small copies cost one hand-off instead of one each.
+    void copyPieces(const Piece* pieces, size_t count) {
+        size_t bytes = 0;
+        for (size_t i = 0; i < count; i++) {
+            bytes += pieces[i].bytes;
+        }
         if (bytes < kParallelMin || m_threads.empty()) {
-            memcpy(dst, src, bytes);
+            for (size_t i = 0; i < count; i++) {
+                memcpy(pieces[i].dst, pieces[i].src, pieces[i].bytes);
+            }
             return;
         }
         std::lock_guard<std::mutex> turn(m_turn);
@@ -1720,15 +1739,15 @@ public:
         const size_t slice = ((bytes + parts - 1) / parts + 63) / 64 * 64;
         {
             std::lock_guard<std::mutex> lock(m_mutex);
-            m_dst = static_cast<uint8_t*>(dst);
-            m_src = static_cast<const uint8_t*>(src);
+            m_pieces = pieces;
+            m_count = count;
             m_bytes = bytes;
             m_slice = slice;
             m_pending = static_cast<int>(m_threads.size());
             m_generation++;
         }
         m_start.notify_all();
-        memcpy(dst, src, std::min(slice, bytes));
+        copyRange(pieces, count, 0, std::min(slice, bytes));
         std::unique_lock<std::mutex> lock(m_mutex);
         m_done.wait(lock, [&] { return m_pending == 0; });
     }
@@ -1755,6 +1774,18 @@ private:
         }
     }
 
+    // Bytes [begin, end) of the pieces taken one after another.
+    static void copyRange(const Piece* pieces, size_t count, size_t begin, size_t end) {
+        size_t at = 0;
+        for (size_t i = 0; i < count && at < end; i++) {
+            const size_t b = std::max(begin, at), e = std::min(end, at + pieces[i].bytes);
+            if (b < e) {
+                memcpy(static_cast<uint8_t*>(pieces[i].dst) + (b - at), static_cast<const uint8_t*>(pieces[i].src) + (b - at), e - b);
+            }
+            at += pieces[i].bytes;
+        }
+    }
+
     void work(size_t part) {
         uint64_t seen = 0;
         std::unique_lock<std::mutex> lock(m_mutex);
@@ -1765,11 +1796,11 @@ private:
             }
             seen = m_generation;
             const size_t begin = std::min(part * m_slice, m_bytes), end = std::min(begin + m_slice, m_bytes);
-            uint8_t* dst = m_dst;
-            const uint8_t* src = m_src;
+            const Piece* pieces = m_pieces;
+            const size_t count = m_count;
             lock.unlock();
             if (end > begin) {
-                memcpy(dst + begin, src + begin, end - begin);
+                copyRange(pieces, count, begin, end);
             }
             lock.lock();
             if (--m_pending == 0) {
@@ -1784,8 +1815,8 @@ private:
     uint64_t m_generation = 0;
     int m_pending = 0;
     bool m_stop = false;
-    uint8_t* m_dst = nullptr;
-    const uint8_t* m_src = nullptr;
+    const Piece* m_pieces = nullptr;
+    size_t m_count = 0;
     size_t m_bytes = 0, m_slice = 0;
 };
 
@@ -2870,9 +2901,91 @@ struct HostArray {
     }
 };
 
+// Queued texts wait in pinned host memory, so that a batch uploads each with one DMA: no copy into
+// the staging chunks and no wait for a chunk between texts. Slabs of kSlab bytes are carved in
+// arrival order and reused once every text in them is done; a text larger than a slab, or one
+// arriving while kMaxSlabs slabs are all in use, waits in pageable memory instead.
+class TextPool {
+ public:
+    struct Block {
+        uint8_t* data = nullptr;
+        int slab = -1;
+    };
+
+    ~TextPool() {
+        for (Slab& s : m_slabs) {
+            (void)hipHostFree(s.base);
+        }
+    }
+
+    Block allocate(size_t n) {
+        const size_t need = (n + 63) / 64 * 64;
+        if (n == 0 || need > kSlab) {
+            return {};
+        }
+        std::lock_guard<std::mutex> lock(m_mutex);
+        if (m_current < 0 || m_slabs[m_current].used + need > kSlab) {
+            m_current = -1;
+            for (size_t i = 0; i < m_slabs.size(); i++) {
+                if (m_slabs[i].live == 0) {
+                    m_slabs[i].used = 0;
+                    m_current = static_cast<int>(i);
+                    break;
+                }
+            }
+            if (m_current < 0) {
+                uint8_t* base = nullptr;
+                if (m_slabs.size() >= kMaxSlabs || hipHostMalloc(reinterpret_cast<void**>(&base), kSlab, hipHostMallocDefault) != hipSuccess) {
+                    (void)hipGetLastError();
+                    return {};
+                }
+                m_slabs.push_back(Slab{base, 0, 0});
+                m_current = static_cast<int>(m_slabs.size()) - 1;
+            }
+        }
+        Slab& s = m_slabs[m_current];
+        Block b{s.base + s.used, m_current};
+        s.used += need;
+        s.live++;
+        return b;
+    }
+
+    void release(Block& b) {
+        if (!b.data) {
+            return;
+        }
+        std::lock_guard<std::mutex> lock(m_mutex);
+        m_slabs[b.slab].live--;
+        b = {};
+    }
+
+ private:
+    static constexpr size_t kSlab = size_t(64) << 20;
+    static constexpr size_t kMaxSlabs = 16;
+    struct Slab {
+        uint8_t* base;
+        size_t used;
+        int live;
+    };
+    std::mutex m_mutex;
+    std::vector<Slab> m_slabs;
+    int m_current = -1;
+};
+
+// A queued text: in a pinned block of the server's pool, or in pageable memory.
+struct RequestText {
+    HostArray<uint8_t> pageable;
+    TextPool::Block pinned;
+    size_t size = 0;
+
+    const uint8_t* ptr() const {
+        return pinned.data ? pinned.data : pageable.data.get();
+    }
+};
+
 struct ServerRequest {
     int64_t id = 0;
-    HostArray<uint8_t> text;
+    RequestText text;
     HostArray<int32_t> sa;
     HostArray<uint8_t> bwt;
     int32_t* outSA = nullptr;   // the caller's buffers, written at completion (submit_into);
@@ -2896,7 +3009,7 @@ class Server {
         tmp.apply(&sp);
         m_params = tmp;
         m_budget = static_cast<size_t>(p.memory_budget);
-        m_workerCount = p.workers > 0 ? p.workers : 3;
+        m_workerCount = p.workers > 0 ? p.workers : 5;
         m_maxBatchBytes = p.max_batch_bytes > 0 ? static_cast<uint64_t>(p.max_batch_bytes) : (uint64_t(32) << 20);
         m_maxBatchTexts = p.max_batch_texts > 0 ? p.max_batch_texts : 4096;
         m_wantSA = p.want_sa != 0;
@@ -2974,9 +3087,15 @@ class Server {
 
     int64_t submit(const uint8_t* text, int64_t n, int32_t* outSA = nullptr, uint8_t* outBWT = nullptr) {
         auto r = std::make_shared<ServerRequest>();
-        r->text.allocate(static_cast<size_t>(n));
+        r->text.size = static_cast<size_t>(n);
+        if (n > 1) {
+            r->text.pinned = m_textPool.allocate(static_cast<size_t>(n));
+        }
+        if (!r->text.pinned.data) {
+            r->text.pageable.allocate(static_cast<size_t>(n));
+        }
         if (n > 0) {
-            HostCopier::instance().copy(r->text.data.get(), text, static_cast<size_t>(n));
+            HostCopier::instance().copy(const_cast<uint8_t*>(r->text.ptr()), text, static_cast<size_t>(n));
         }
         r->outSA = m_wantSA ? outSA : nullptr;
         r->outBWT = m_wantBWT ? outBWT : nullptr;
@@ -3064,7 +3183,13 @@ class Server {
         return planFor(planner, layout, mode(), true, true);
     }
 
+    void releaseText(ServerRequest& r) {
+        m_textPool.release(r.text.pinned);
+        r.text.pageable.allocate(0);
+    }
+
     void finish(const std::shared_ptr<ServerRequest>& r, int status, const char* error) {
+        releaseText(*r);
         r->status = status;
         r->error = error ? error : "";
         r->done = std::chrono::steady_clock::now();
@@ -3140,7 +3265,7 @@ class Server {
                             (r->outSA ? r->outSA : r->sa.data.get())[0] = 0;
                         }
                         if (m_wantBWT) {
-                            (r->outBWT ? r->outBWT : r->bwt.data.get())[0] = r->text.data[0];
+                            (r->outBWT ? r->outBWT : r->bwt.data.get())[0] = r->text.ptr()[0];
                         }
                         r->primary = 1;
                     } else {
@@ -3246,7 +3371,7 @@ class Server {
         uint64_t total = 0;
         for (auto& r : batch) {
             lengths.push_back(static_cast<int64_t>(r->text.size));
-            texts.push_back(r->text.data.get());
+            texts.push_back(r->text.ptr());
             total += r->text.size;
         }
         const Layout layout = layoutFor(m_params, lengths.data(), static_cast<int32_t>(lengths.size()), batch.size() > 1);
@@ -3293,6 +3418,9 @@ class Server {
         w.ctx->arenaBytes = 0;
         const std::string error = status == SADCS_OK ? "" : tLastError;
         const auto copyStart = std::chrono::steady_clock::now();
+        // Every request's outputs leave the batch buffer in one parallel copy.
+        std::vector<HostCopier::Piece> pieces;
+        pieces.reserve(2 * batch.size());
         uint64_t offset = 0;
         for (size_t i = 0; i < batch.size(); i++) {
             auto& r = batch[i];
@@ -3304,18 +3432,21 @@ class Server {
                     if (!r->outSA) {
                         r->sa.allocate(n);
                     }
-                    HostCopier::instance().copy(r->outSA ? r->outSA : r->sa.data.get(), sa + offset, n * sizeof(int32_t));
+                    pieces.push_back({r->outSA ? r->outSA : r->sa.data.get(), sa + offset, n * sizeof(int32_t)});
                 }
                 if (m_wantBWT) {
                     if (!r->outBWT) {
                         r->bwt.allocate(n);
                     }
-                    HostCopier::instance().copy(r->outBWT ? r->outBWT : r->bwt.data.get(), bwt + offset, n);
+                    pieces.push_back({r->outBWT ? r->outBWT : r->bwt.data.get(), bwt + offset, n});
                     r->primary = primaries[i];
                 }
             }
             offset += n;
-            r->text.allocate(0);
+        }
+        HostCopier::instance().copyPieces(pieces.data(), pieces.size());
+        for (auto& r : batch) {
+            releaseText(*r);
         }
         std::lock_guard<std::mutex> lock(m_statsMutex);
         m_stats.batches++;
@@ -3350,6 +3481,7 @@ class Server {
     std::condition_variable m_wake, m_done;
     std::deque<std::shared_ptr<ServerRequest>> m_pending;
     std::unordered_map<int64_t, std::shared_ptr<ServerRequest>> m_requests;
+    TextPool m_textPool;  // queued texts, pinned
     std::vector<std::unique_ptr<Worker>> m_workers;
     std::thread m_dispatcher;
     int64_t m_nextId = 0, m_completed = 0, m_batches = 0, m_maxConcurrent = 0;
