@@ -215,7 +215,8 @@ typedef struct sadcs_server sadcs_server;
 
 int sadcs_hip_server_create(const sadcs_server_params* params, sadcs_server** server);
 
-/* Copies the text and queues it; returns the request id (>= 0) or minus a status. */
+/* Copies the text (into the server's pinned host slabs while they have room, 1 GiB at most) and
+ * queues it; returns the request id (>= 0) or minus a status. */
 int64_t sadcs_hip_server_submit(sadcs_server* server, const uint8_t* text, int64_t n);
 
 /* As sadcs_hip_server_submit, with the outputs written straight into the caller's buffers (sa: n
