@@ -405,3 +405,25 @@ def test_serving_submit_into_with_one_output(gpu):
             assert np.array_equal(sa, want)
             assert (wait_bwt.tobytes(), primary.value) == oracle.bwt_from_sa(t.tobytes(), want)
             assert (wait_sa == -9).all(), "wait wrote an output that was given to submit_into"
+
+
+def test_serving_texts_in_and_beyond_the_pinned_slabs(gpu):
+    """Queued texts wait in 64 MiB pinned slabs that are reused once their texts are done; a text
+    larger than a slab waits in pageable memory. Two waves of 3 MiB texts (more than one slab per
+    wave, so slabs fill and are reused by the second wave) and one 65 MiB text: every result
+    equals the reference build's."""
+    from sadcs_amd import workloads
+    waves = [[workloads.markov(3 << 20, seed=60 + 20 * w + i).tobytes() for i in range(20)] for w in range(2)]
+    big = workloads.dna_like((65 << 20) + 17, seed=59).tobytes()
+    texts = waves[0] + waves[1] + [big]
+    want = oracle.ref_outputs(texts)
+    results = []
+    with gpu.Server(workers=2) as server:
+        for wave in waves:
+            ids = [server.submit(t) for t in wave]
+            results += [server.wait(i) for i in ids]
+        results.append(server.wait(server.submit(big)))
+    for i, (got, (sa, b, primary)) in enumerate(zip(results, want)):
+        print(f"request {i}: {len(texts[i])} bytes", flush=True)
+        assert np.array_equal(got[0], sa), f"request {i}: SA differs from the reference build"
+        assert (got[1], got[2]) == (b, primary), f"request {i}: BWT differs from the reference build"
