@@ -406,6 +406,9 @@ def main():
     ap.add_argument("--config-lines", type=int, default=1, help="secondary lines: the other cover, C1 and C2 (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES device to device three times (the known-size kernel that scales HBM counters)")
+    ap.add_argument("--gather-calibration", type=int, default=0, metavar="GATHERS",
+                    help="first gather GATHERS random 128-byte lines of a 4 GiB array three times, whole and as the "
+                         "merge reads its records (known-size random reads that scale HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -439,6 +442,16 @@ def main():
         source.free()
         target.free()
         print(f"# copied {args.copy_calibration} bytes device to device 3 times", file=sys.stderr, flush=True)
+
+    if args.gather_calibration:
+        lines = sadcs_amd.DeviceBuffer(4 << 30)
+        lines.zero()
+        for whole in (True, False):
+            for _ in range(3):
+                sadcs_amd.gpu_gather_lines(lines.ptr, (4 << 30) // 128, args.gather_calibration, whole)
+        lines.free()
+        print(f"# gathered {args.gather_calibration} random lines of a 4 GiB array 3 times whole, then 3 times as records",
+              file=sys.stderr, flush=True)
 
     tuned, tune_seconds = {}, 0.0
     if args.tune_mib:

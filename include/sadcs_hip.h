@@ -193,6 +193,12 @@ int sadcs_hip_memcpy(void* dst, const void* src, size_t bytes, int32_t kind);
 int sadcs_hip_memset(void* dst, int32_t value, size_t bytes);
 int sadcs_hip_synchronize(void);
 
+/* A known-size random gather for scaling the HBM counters of a profiler (no counterpart in the
+ * reference): `gathers` reads of random 128-byte lines of d_lines (line_count lines, at most
+ * 2^32 - 1): whole lines, or (whole_lines = 0) 96 bytes of each as the top-level merge reads its
+ * records. Synchronous; kernel `gatherLinesKernel`. */
+int sadcs_hip_gather_lines(const void* d_lines, int64_t line_count, int64_t gathers, int32_t whole_lines);
+
 /* ---- serving: a request loop with continuous batching (no counterpart in the reference) ----
  * Submitted texts queue; whenever one of `workers` device streams is idle, a batch is formed from
  * the front of the queue (at most max_batch_bytes of text and max_batch_texts texts; from a deep

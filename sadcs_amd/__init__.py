@@ -137,6 +137,8 @@ def _load():
     lib.sadcs_hip_check_device.restype = ctypes.c_int
     lib.sadcs_hip_check_device_noalloc.argtypes = [P, i32, P, P, i32, ctypes.POINTER(i64)]
     lib.sadcs_hip_check_device_noalloc.restype = ctypes.c_int
+    lib.sadcs_hip_gather_lines.argtypes = [P, i64, i64, i32]
+    lib.sadcs_hip_gather_lines.restype = ctypes.c_int
     return lib
 
 
@@ -151,7 +153,7 @@ EXPORTED_SYMBOLS = (
     "sadcs_hip_server_create", "sadcs_hip_server_submit", "sadcs_hip_server_submit_into", "sadcs_hip_server_wait", "sadcs_hip_server_counters",
     "sadcs_hip_server_destroy", "sadcs_hip_ctx_create_batch", "sadcs_hip_device_required_bytes", "sadcs_hip_tune", "sadcs_hip_device_alloc", "sadcs_hip_device_free", "sadcs_hip_memcpy",
     "sadcs_hip_memset", "sadcs_hip_synchronize", "sadcs_hip_check_device", "sadcs_hip_check_device_noalloc",
-    "sadcs_hip_server_stats",
+    "sadcs_hip_server_stats", "sadcs_hip_gather_lines",
 )
 
 
@@ -480,6 +482,12 @@ def gpu_synchronize():
 
 def gpu_copy_device(dst, src, size):
     _check(lib.sadcs_hip_memcpy(dst, src, int(size), 3))
+
+
+def gpu_gather_lines(d_lines, line_count, gathers, whole_lines=True):
+    """sadcs_hip_gather_lines: `gathers` reads of random 128-byte lines (whole, or 96 bytes of each
+    as the top-level merge reads records), the known-size kernel for scaling HBM counters."""
+    _check(lib.sadcs_hip_gather_lines(ctypes.c_void_p(d_lines), int(line_count), int(gathers), int(bool(whole_lines))))
 
 
 CHECK_NAMES = {0: "pass", 1: "entry out of range", 2: "duplicate entry", 3: "SA[index] and SA[index + 1] out of order",

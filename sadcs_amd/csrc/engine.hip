@@ -2290,6 +2290,40 @@ int buildOn(sadcs_ctx* context, const Params& params, const Layout& layout, cons
 
 using namespace sadcs;
 
+namespace sadcs {
+
+// Gathers of random 128-byte lines of a lines array: whole lines (eight lanes, 16 bytes each), or
+// as the top-level merge reads its records (a lane pair per line: one lane three quads from the
+// first half, the other three from the second). Each thread writes the XOR of what it loaded.
+__global__ __launch_bounds__(256) void gatherLinesKernel(const uint4* __restrict__ lines, uint32_t lineCount, uint64_t gathers,
+                                                         int wholeLines, uint32_t* __restrict__ out) {
+    const uint64_t thread = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint32_t lanes = wholeLines ? 8 : 2;
+    const uint32_t part = static_cast<uint32_t>(thread % lanes);
+    const uint64_t slots = static_cast<uint64_t>(gridDim.x) * blockDim.x / lanes;
+    uint32_t acc = 0;
+    for (uint64_t g = thread / lanes; g < gathers; g += slots) {
+        uint32_t h = static_cast<uint32_t>(g) * 0x9E3779B9u ^ static_cast<uint32_t>(g >> 32);
+        h ^= h >> 16;
+        h *= 0x85EBCA6Bu;
+        h ^= h >> 13;
+        h *= 0xC2B2AE35u;
+        h ^= h >> 16;
+        const uint4* line = lines + (static_cast<uint64_t>(h) * lineCount >> 32) * 8;
+        if (wholeLines) {
+            const uint4 v = line[part];
+            acc ^= v.x ^ v.y ^ v.z ^ v.w;
+        } else {
+            const uint4* from = line + (part ? 4 : 0) + (h & 1);
+            const uint4 a = from[0], b = from[1], c = from[2];
+            acc ^= a.x ^ a.w ^ b.y ^ b.z ^ c.x ^ c.w;
+        }
+    }
+    out[thread] = acc;
+}
+
+}  // namespace sadcs
+
 extern "C" {
 
 int sadcs_hip_query(sadcs_device* device) {
@@ -2622,6 +2656,24 @@ int sadcs_hip_memset(void* dst, int32_t value, size_t bytes) {
 
 int sadcs_hip_synchronize(void) {
     SADCS_HIP(hipDeviceSynchronize());
+    return SADCS_OK;
+}
+
+int sadcs_hip_gather_lines(const void* d_lines, int64_t line_count, int64_t gathers, int32_t whole_lines) {
+    if (!d_lines || line_count <= 0 || line_count > INT64_C(0xFFFFFFFF) || gathers < 0) {
+        setError("gather_lines: bad arguments");
+        return SADCS_FAILED;
+    }
+    constexpr unsigned kBlocks = 2048, kThreads = 256;
+    uint32_t* out = nullptr;
+    SADCS_HIP(hipMalloc(&out, sizeof(uint32_t) * kBlocks * kThreads));
+    sadcs::gatherLinesKernel<<<kBlocks, kThreads>>>(static_cast<const uint4*>(d_lines), static_cast<uint32_t>(line_count),
+                                                    static_cast<uint64_t>(gathers), whole_lines, out);
+    const hipError_t launched = hipGetLastError();
+    const hipError_t done = hipDeviceSynchronize();
+    (void)hipFree(out);
+    SADCS_HIP(launched);
+    SADCS_HIP(done);
     return SADCS_OK;
 }
 
