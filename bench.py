@@ -204,9 +204,13 @@ def batched_measurement(count, steps, tune=True, cpu_texts=0):
     sadcs_amd.gpu_synchronize()
     start = time.perf_counter()
     for _ in range(steps):
-        batch_primaries = sadcs_amd.gpu_batch_build_device(batch_ctx, d_texts.ptr, lengths, d_sa.ptr, d_bwt.ptr)
+        batch_primaries = sadcs_amd.gpu_batch_build_device(batch_ctx, d_texts.ptr, lengths, d_sa.ptr, d_bwt.ptr, stats)
     sadcs_amd.gpu_synchronize()
     batch_seconds = (time.perf_counter() - start) / steps
+    phases = {"ranking": round(stats.ranking_seconds * 1e3, 2), "classes": round(stats.class_sort_seconds * 1e3, 2),
+              "merge": round(stats.merge_seconds * 1e3, 2), "merge_kernel": round(stats.merge_kernel_seconds * 1e3, 2)}
+    print(f"# batched: {batch_seconds * 1e3:.1f} ms per batch (last one: {phases}; "
+          f"{stats.character_passes} character passes, {stats.doubling_rounds} doubling rounds)", file=sys.stderr, flush=True)
 
     seq_primaries = np.empty(count, dtype=np.int64)
     sadcs_amd.gpu_synchronize()
@@ -230,7 +234,7 @@ def batched_measurement(count, steps, tune=True, cpu_texts=0):
             "ms": round(batch_seconds * 1e3, 2), "sequential_mb_per_s": round(total / seq_seconds / 1e6, 1),
             "sequential_ms": round(seq_seconds * 1e3, 1), "speedup_vs_sequential": round(seq_seconds / batch_seconds, 2),
             "outputs_equal_sequential": same, "resident": "texts in HBM, outputs in HBM",
-            "merge_params": tuned or "defaults", "cpu_baseline": cpu}
+            "phases_ms": phases, "merge_params": tuned or "defaults", "cpu_baseline": cpu}
 
 
 def served_measurement(requests, factors=(0.5, 0.8, 1.1, 2.0, 3.0, 4.0, None), cpu=False, workers=0, batch_mib=0):
