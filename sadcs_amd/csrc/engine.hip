@@ -2970,6 +2970,19 @@ class TextPool {
         }
     }
 
+    // Pins `count` slabs now (pinning is slow: better at start than while texts arrive).
+    void reserve(size_t count) {
+        std::lock_guard<std::mutex> lock(m_mutex);
+        while (m_slabs.size() < std::min(count, kMaxSlabs)) {
+            uint8_t* base = nullptr;
+            if (hipHostMalloc(reinterpret_cast<void**>(&base), kSlab, hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();
+                return;
+            }
+            m_slabs.push_back(Slab{base, 0, 0});
+        }
+    }
+
     Block allocate(size_t n) {
         const size_t need = (n + 63) / 64 * 64;
         if (n == 0 || need > kSlab) {
@@ -3098,6 +3111,18 @@ class Server {
             m_workers.push_back(std::make_unique<Worker>());
             m_workers.back()->ctx = ctx;
         }
+        // Host memory is pinned now rather than while requests arrive: text slabs for the queue, and
+        // each worker's output buffer for a full batch (grown later only for a larger lone text).
+        m_textPool.reserve(kReservedTextSlabs);
+        const size_t outputBytes = static_cast<size_t>(m_maxBatchBytes) * ((m_wantSA ? sizeof(int32_t) : 0) + (m_wantBWT ? 1 : 0));
+        for (auto& w : m_workers) {
+            if (outputBytes && hipHostMalloc(reinterpret_cast<void**>(&w->pinnedOut), outputBytes, hipHostMallocDefault) == hipSuccess) {
+                w->pinnedBytes = outputBytes;
+            } else {
+                (void)hipGetLastError();
+                w->pinnedOut = nullptr;
+            }
+        }
         for (int w = 0; w < m_workerCount; w++) {
             m_workers[w]->thread = std::thread(&Server::workerLoop, this, w);
         }
@@ -3146,8 +3171,12 @@ class Server {
         if (!r->text.pinned.data) {
             r->text.pageable.allocate(static_cast<size_t>(n));
         }
-        if (n > 0) {
+        // A text below a few MiB is copied by this thread: the copy pool is shared with the workers'
+        // output copies, and waiting for its turn would hold up the submitting thread.
+        if (n >= kPooledTextCopy) {
             HostCopier::instance().copy(const_cast<uint8_t*>(r->text.ptr()), text, static_cast<size_t>(n));
+        } else if (n > 0) {
+            memcpy(const_cast<uint8_t*>(r->text.ptr()), text, static_cast<size_t>(n));
         }
         r->outSA = m_wantSA ? outSA : nullptr;
         r->outBWT = m_wantBWT ? outBWT : nullptr;
@@ -3534,6 +3563,8 @@ class Server {
     std::deque<std::shared_ptr<ServerRequest>> m_pending;
     std::unordered_map<int64_t, std::shared_ptr<ServerRequest>> m_requests;
     TextPool m_textPool;  // queued texts, pinned
+    static constexpr int64_t kPooledTextCopy = int64_t(8) << 20;
+    static constexpr size_t kReservedTextSlabs = 4;
     std::vector<std::unique_ptr<Worker>> m_workers;
     std::thread m_dispatcher;
     int64_t m_nextId = 0, m_completed = 0, m_batches = 0, m_maxConcurrent = 0;
