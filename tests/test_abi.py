@@ -69,3 +69,12 @@ def test_device_call_memory_plan():
     for n in (1 << 20, 1 << 26, 1 << 30, (1 << 31) - 1025):
         own = sadcs_amd.lib.sadcs_hip_device_required_bytes(n, 1, 1)
         assert 0 < own <= 4 * n + (64 << 10), (n, own / n)
+
+
+def test_gather_calibration_refuses_bad_arguments():
+    """sadcs_hip_gather_lines checks its arguments before any device work."""
+    lib = sadcs_amd.lib
+    assert lib.sadcs_hip_gather_lines(None, 1024, 10, 1) == sadcs_amd.GpuStatus.Failed
+    assert lib.sadcs_hip_gather_lines(ctypes.c_void_p(16), 0, 10, 1) == sadcs_amd.GpuStatus.Failed
+    assert lib.sadcs_hip_gather_lines(ctypes.c_void_p(16), 1 << 32, 10, 1) == sadcs_amd.GpuStatus.Failed
+    assert "gather_lines" in sadcs_amd.gpu_last_error()

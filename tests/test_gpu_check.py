@@ -164,3 +164,12 @@ def test_largest_text_passes_the_device_check(gpu):
     # at the peak), and a few KiB of fixed tables (text padding, segment table, error flags).
     assert stats.device_bytes <= 10 * n + (64 << 10)
     assert gpu.gpu_check_device(d_text.ptr, n, d_sa.ptr, d_bwt.ptr, primary) == (0, -1)
+
+
+def test_gather_calibration_runs(gpu):
+    """The known-size line gather used to scale HBM counters runs in both modes on a small array."""
+    lines = gpu.DeviceBuffer(1 << 20)
+    lines.zero()
+    for whole in (True, False):
+        gpu.gpu_gather_lines(lines.ptr, (1 << 20) // 128, 100_000, whole)
+    lines.free()
