@@ -2,7 +2,8 @@
 downloaded while later slices merge (more than 2^24 output rows).
 
 Whatever the slicing, a host call must return exactly what a device-resident build leaves in HBM
-(itself checked row by row on the device): for the SA and the BWT together and alone, into
+(itself checked row by row on the device, and against the reference CPU build for the 40 MiB
+text): for the SA and the BWT together and alone, into
 pageable and into pinned host memory, for cover 32 and cover 128, and for a batch of texts whose
 slices cut across text boundaries (the reference's host API: src/cuda/gpusa.h:101-108).
 """
@@ -10,6 +11,8 @@ import ctypes
 
 import numpy as np
 import pytest
+
+import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -30,6 +33,11 @@ def reference_build(gpu):
     sa, bwt = d_sa.to_host(np.int32), d_bwt.to_host(np.uint8).tobytes()
     for buf in (d_text, d_sa, d_bwt):
         buf.free()
+    # And equal to the reference CPU build of the same text, so the host calls below are compared
+    # with the reference as well.
+    want_sa = oracle.ref_suffix_array(text[:N].tobytes())
+    assert np.array_equal(sa, want_sa)
+    assert (bwt, primary) == oracle.bwt_from_sa(text[:N].tobytes(), want_sa)
     return text[:N].copy(), sa, bwt, primary
 
 
@@ -82,17 +90,21 @@ def test_cover_128_context(gpu, reference_build):
 
 def test_batch_slices_cut_across_texts(gpu):
     """A 48 MiB batch of 23 texts: slice boundaries fall inside texts, each text's BWT shifted by
-    its own first row. Every text's outputs equal its single-text build."""
+    its own first row. Every text's outputs equal its single-text build and the reference CPU
+    build's."""
     from sadcs_amd import workloads
     rng = np.random.default_rng(11)
     lengths = [int(x) for x in rng.integers(1 << 20, 3 << 20, 23)]
     texts = [workloads.markov(n, 200 + i) if i % 3 else workloads.dna_like(n, 300 + i) for i, n in enumerate(lengths)]
     print(f"batch of {len(texts)} texts, {sum(lengths)} bytes", flush=True)
     results = gpu.gpu_batch_build(texts)
-    for i, (t, (sa, bwt, primary)) in enumerate(zip(texts, results)):
+    reference = oracle.ref_outputs([t.tobytes() for t in texts])
+    for i, (t, (sa, bwt, primary), (ref_sa, ref_bwt, ref_primary)) in enumerate(zip(texts, results, reference)):
         want_sa, want_bwt, want_primary = gpu.gpu_build(t)
         assert np.array_equal(sa, want_sa), i
         assert bwt == want_bwt and primary == want_primary, i
+        assert np.array_equal(sa, ref_sa), i
+        assert (bwt, primary) == (ref_bwt, ref_primary), i
 
 
 def test_two_slices_odd_length(gpu):
