@@ -370,7 +370,8 @@ def device_line(sadcs_amd, d_text, n, d_sa, d_bwt, cover, steps, warmup, label):
             "mb_per_s": round(n / step / 1e6, 1), "e2e_hbm_fraction": round(BYTES_PER_INPUT_BYTE * n / step / HBM_PEAK, 4),
             "phases_ms": {"ranking": round(ranking * 1e3, 2), "classes": round(classes * 1e3, 2), "merge": round(merge * 1e3, 2)},
             "character_passes": stats.character_passes, "doubling_rounds": stats.doubling_rounds,
-            "merge_kernel_ms": round(stats.merge_kernel_seconds * 1e3, 3), "output_check": "pass (device, every row)"}
+            "merge_kernel_ms": round(stats.merge_kernel_seconds * 1e3, 3),
+            "peak_device_bytes_per_input_byte": round(stats.device_bytes / n, 3), "output_check": "pass (device, every row)"}
     print(f"# {label}, cover {cover}: {line}", file=sys.stderr, flush=True)
     return line
 

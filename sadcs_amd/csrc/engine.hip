@@ -412,16 +412,13 @@ struct Params {
     uint32_t k = 0;         // sorted splitters per tile; 0: the most that keep the CU's tiles in its LDS
     int threads = kMergeThreads;  // threads per merge tile (512 or 256)
 
+    // Zero fields take the defaults of the cover.
     void apply(const sadcs_params* p) {
-        if (p && p->tile_threads) {
-            threads = static_cast<int>(p->tile_threads);
-        }
         if (p && p->cover) {
             cover = p->cover;
         }
-        if (p && p->splitter_spacing) {
-            g = p->splitter_spacing;
-        }
+        threads = p && p->tile_threads ? static_cast<int>(p->tile_threads) : defaultTileThreads(cover);
+        g = p && p->splitter_spacing ? p->splitter_spacing : defaultSplitterSpacing(cover);
         if (p && p->tile_splitters) {
             k = p->tile_splitters;
         }
@@ -2395,8 +2392,10 @@ int sadcs_hip_tune(const void* d_texts, const int64_t* lengths, int32_t count, c
         }
         return s;
     };
+    // Cover 128's 31 lists want smaller spacings than cover 32's 7 (the spill room of a tile).
+    const std::vector<uint32_t> spacings = cover == 128 ? std::vector<uint32_t>{3, 4, 6, 8} : std::vector<uint32_t>{8, 12, 16, 24, 32};
     for (int threads : {512, 256}) {
-        for (uint32_t g : {8u, 12u, 16u, 24u, 32u}) {
+        for (uint32_t g : spacings) {
             const uint32_t most = cover == 128 ? (batch ? defaultTileSplitters<128, true>(g, threads) : defaultTileSplitters<128, false>(g, threads))
                                                : (batch ? defaultTileSplitters<32, true>(g, threads) : defaultTileSplitters<32, false>(g, threads));
             for (uint32_t k : {most, std::max(1u, 3 * most / 4), std::max(1u, most / 2)}) {

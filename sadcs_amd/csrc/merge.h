@@ -626,6 +626,19 @@ struct TileOut {
 // Threads per tile: 512 (two tiles per CU in the LDS) or 256 (four tiles per CU, each with half the
 // records), a tuned parameter. kMergeThreads is the default.
 constexpr int kMergeThreads = 512;
+
+// The defaults per cover, measured on one MI355X (16 MiB to 1 GiB texts). A tile holds up to
+// K*G + C*(G-1) records for C lists, C*(G-1) of them the room its unaligned run ends need, so the
+// 31 lists of cover 128 want a small G: at G = 16 that room is 465 of a 512-thread tile's 865
+// records. Cover 128 merges in 256-thread tiles at G = 4 (four tiles per CU, 429 records each),
+// 1.9 times faster than in 512-thread tiles at G = 16; cover 32 (7 lists) in 512-thread tiles
+// at G = 16.
+constexpr uint32_t defaultSplitterSpacing(int cover) {
+    return cover == 128 ? 4 : 16;
+}
+constexpr int defaultTileThreads(int cover) {
+    return cover == 128 ? 256 : kMergeThreads;
+}
 constexpr int kLoadSlots = 3;  // records each thread gathers at once (default tiles hold at most 3 * threads)
 
 // A tile's records in LDS as a structure of arrays: a wave reading one field of 64 random records

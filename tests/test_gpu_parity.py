@@ -224,6 +224,20 @@ def test_merge_parameters(gpu):
             check_case(gpu, f"merge g={g} k={k} threads={threads}", text, ctx)
 
 
+def test_merge_parameters_cover_128(gpu):
+    """Cover 128 (31 lists) with its default tiles (G = 4, 256 threads), the cover-32 defaults
+    (G = 16, 512 threads) and small tiles that force several merge levels."""
+    rng = oracle.Rng(19)
+    text = rng.random_text(400_000, 4)
+    for g, k, threads in [(0, 0, 0), (16, 0, 512), (2, 1, 256), (3, 2, 512), (4, 1, 256), (8, 0, 256)]:
+        with gpu.GpuContext(len(text), cover=128, splitter_spacing=g, tile_splitters=k, tile_threads=threads) as ctx:
+            check_case(gpu, f"cover 128 merge g={g} k={k} threads={threads}", text, ctx)
+            sa, b, primary = gpu.gpu_build(text, True, True, context=ctx)  # the fused merge
+            want = expected_sa(text)
+            assert np.array_equal(sa, want)
+            assert (b, primary) == oracle.bwt_from_sa(text, want)
+
+
 def test_batch_equals_single(gpu):
     rng = oracle.Rng(13)
     texts = [b"banana", b"a", b"ab" * 50, rng.random_text(3000, 4), rng.periodic_text(7000, 33),
