@@ -43,7 +43,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
 BYTES_PER_INPUT_BYTE = 127.94   # fused SA + BWT, the reference design's logical traffic
-MERGE_BYTES_PER_ELEMENT = 54.0  # list entry 4 + record 44 + SA 4 + BWT 2
+# Top-level merge, per element: list entry 4 + record (16 characters and a 4-byte rank per sample
+# of its window: 7 for cover 32, 13 for cover 128) + SA 4 + BWT 2.
+MERGE_BYTES_PER_ELEMENT = {32: 54.0, 128: 78.0}
 METRIC = "SA+BWT input MB/s at 1 GiB; batched/served MB/s + p99 vs sequential"
 
 
@@ -492,7 +494,7 @@ def main():
     step = elapsed / args.steps
     value = n / step / 1e6
     kernel = sum(merge_kernel) / len(merge_kernel)
-    achieved = MERGE_BYTES_PER_ELEMENT * n / kernel
+    achieved = MERGE_BYTES_PER_ELEMENT[args.cover] * n / kernel
     e2e = BYTES_PER_INPUT_BYTE * n / step / HBM_PEAK
 
     # The timed output checked whole on the device: a permutation, neighbours in order (the
@@ -542,8 +544,9 @@ def main():
                    "merge_params": tuned or "defaults", "tuning_seconds": round(tune_seconds, 2)},
         "roofline": {"bound": "hbm", "kernel": "mergeTilesKernel (top level, SA+BWT in place)",
                      "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": measured_traffic(args.config, n),
-                     "alg_bytes_per_launch": int(MERGE_BYTES_PER_ELEMENT * n), "avg_launch_ms": round(kernel * 1e3, 3)},
+                     "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": measured_traffic(args.config, n) if args.cover == 32 else None,  # measured at cover 32
+                     "alg_bytes_per_launch": int(MERGE_BYTES_PER_ELEMENT[args.cover] * n), "avg_launch_ms": round(kernel * 1e3, 3)},
         "e2e_hbm_fraction": round(e2e, 4),
         "phases_ms": {"ranking": round(ranking * 1e3, 2), "classes": round(classes * 1e3, 2), "merge": round(merge * 1e3, 2)},
         "peak_device_bytes_per_input_byte": round(peak_bytes / n, 3),
