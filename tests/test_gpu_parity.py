@@ -251,6 +251,32 @@ def test_batch_equals_single(gpu):
         assert (b, primary) == oracle.bwt_from_sa(text, want), f"batch text {i}: BWT differs"
 
 
+def test_batch_cover_128(gpu):
+    """A batch at cover 128 (31 lists with the text index in the keys), with its default tiles and
+    with small ones, and the server at cover 128: every text equals the oracle's."""
+    rng = oracle.Rng(23)
+    texts = [b"banana", b"a", b"ab" * 70, rng.random_text(5000, 4), rng.periodic_text(9000, 129),
+             oracle.fibonacci_text(20000), rng.random_text(300, 256), b"\0" * 500, rng.random_text(150000, 20)]
+    lengths = [len(t) for t in texts]
+    for g, k, threads in [(0, 0, 0), (2, 1, 512)]:
+        print(f"cover 128 batch g={g} k={k} threads={threads}", flush=True)
+        with gpu.GpuContext(0, cover=128, splitter_spacing=g, tile_splitters=k, batch_lengths=lengths, tile_threads=threads) as ctx:
+            results = gpu.gpu_batch_build(texts, context=ctx)
+        for i, text in enumerate(texts):
+            sa, b, primary = results[i]
+            want = expected_sa(text)
+            assert np.array_equal(sa, want), f"cover 128 batch text {i}: SA differs"
+            assert (b, primary) == oracle.bwt_from_sa(text, want), f"cover 128 batch text {i}: BWT differs"
+    print("cover 128 server", flush=True)
+    with gpu.Server(workers=2, cover=128) as server:
+        handles = [server.submit(t) for t in texts]
+        for i, (text, h) in enumerate(zip(texts, handles)):
+            sa, b, primary = server.wait(h)[:3]
+            want = expected_sa(text)
+            assert np.array_equal(sa, want), f"cover 128 served text {i}: SA differs"
+            assert (b, primary) == oracle.bwt_from_sa(text, want), f"cover 128 served text {i}: BWT differs"
+
+
 def test_determinism(gpu):
     text = oracle.Rng(17).random_text(500_000, 4)
     a = gpu.gpu_build(text)
