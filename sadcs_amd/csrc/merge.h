@@ -316,10 +316,13 @@ __device__ __forceinline__ bool lessThanRecord(const MergeCtx& c, const Texts& t
     return c.ranks[packPosition<V>(i + k)] < rankB;
 }
 
+// The list holding concatenated index `index`: the number of list starts after the first at or
+// below it. Branch-free, against bases that are the same for every lane (kernel arguments).
 __device__ __forceinline__ int listOfIndex(const ListSet& lists, uint32_t index) {
     int l = 0;
-    while (l + 1 < lists.count && index >= lists.indexBase[l + 1]) {
-        l++;
+#pragma unroll
+    for (int k = 1; k < kMaxLists; k++) {
+        l += k < lists.count && index >= lists.indexBase[k] ? 1 : 0;
     }
     return l;
 }
@@ -336,27 +339,35 @@ __global__ void gatherSplittersKernel(ListSet lists, ListSet splitters, uint32_t
 }
 
 // counts[l * boundaries + b] = splitters of list l among sorted splitters [b*K, (b+1)*K). One
-// wave per boundary: the lanes classify 64 splitters at a time and one ballot per list counts them.
+// wave per boundary: the lanes classify 64 splitters at a time into the wave's LDS histogram.
 __global__ __launch_bounds__(256) void countBoundarySplittersKernel(ListSet splitters, const uint32_t* __restrict__ sorted, uint32_t total, uint32_t perTile, uint32_t boundaries, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t histogram[256 / kWave][kMaxLists];
     const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const uint32_t lane = threadIdx.x % kWave;
+    const uint32_t wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     if (b >= boundaries) {
-        return;  // whole waves leave together
+        return;  // whole waves leave together, and no wave shares its histogram
     }
     const int lc = splitters.count;
+    uint32_t* h = histogram[wave];
+    if (lane < static_cast<uint32_t>(kMaxLists)) {
+        h[lane] = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const uint64_t first = static_cast<uint64_t>(b) * perTile;
     const uint64_t last = min(first + perTile, static_cast<uint64_t>(total));
-    uint32_t mine = 0;  // lane l < lc: the count of list l
     for (uint64_t x0 = first; x0 < last; x0 += kWave) {
         const uint64_t x = x0 + lane;
-        const int l = x < last ? listOfIndex(splitters, sorted[x]) : -1;
-        for (int k = 0; k < lc; k++) {
-            const uint32_t c = static_cast<uint32_t>(__builtin_popcountll(ballot(l == k)));
-            mine += lane == static_cast<uint32_t>(k) ? c : 0;
+        if (x < last) {
+            atomicAdd(&h[listOfIndex(splitters, sorted[x])], 1u);
         }
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (lane < static_cast<uint32_t>(lc)) {
-        counts[static_cast<uint64_t>(lane) * boundaries + b] = mine;
+        counts[static_cast<uint64_t>(lane) * boundaries + b] = h[lane];
     }
 }
 
