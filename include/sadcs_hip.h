@@ -64,7 +64,7 @@ typedef struct sadcs_stats {
 /* Construction parameters; zero fields take the defaults (of the cover, for G and tile threads). */
 typedef struct sadcs_params {
     int32_t cover;               /* 32 (default) or 128 */
-    uint32_t splitter_spacing;   /* G: every G-th element of a list is a splitter; 16 (cover 32), 4 (cover 128) */
+    uint32_t splitter_spacing;   /* G: every G-th element of a list is a splitter; 16 (cover 32), 8 (cover 128) */
     uint32_t tile_splitters;     /* K: sorted splitters per merge tile; the most that fit the tiles in LDS */
     uint32_t tile_threads;       /* threads per merge tile: 512 (two tiles per CU; cover 32) or 256 (four; cover 128) */
 } sadcs_params;
@@ -146,7 +146,7 @@ int sadcs_hip_batch_build_device(sadcs_ctx* context, const void* d_texts, const 
 
 /* ---- autotuner (no counterpart in the reference) ----
  * Builds SA + BWT of a device-resident sample with each candidate merge configuration - tile
- * threads 512 or 256, splitter spacing G in {8, 12, 16, 24, 32} (cover 128: {3, 4, 6, 8}) and tile splitters K at 1, 3/4 and
+ * threads 512 or 256, splitter spacing G in {8, 12, 16, 24, 32} (cover 128: {4, 6, 8, 12}) and tile splitters K at 1, 3/4 and
  * 1/2 of the most that keep the tiles a CU runs at once in its LDS - and writes the fastest (by device time, best of two builds) to *best; the
  * cover comes from base (null: 32). count 1: d_texts is one text of lengths[0] bytes followed by
  * 64 zero bytes; count > 1: a batch in the sadcs_hip_batch_layout layout. best_seconds may be null. */

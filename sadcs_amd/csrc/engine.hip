@@ -2393,7 +2393,7 @@ int sadcs_hip_tune(const void* d_texts, const int64_t* lengths, int32_t count, c
         return s;
     };
     // Cover 128's 31 lists want smaller spacings than cover 32's 7 (the spill room of a tile).
-    const std::vector<uint32_t> spacings = cover == 128 ? std::vector<uint32_t>{3, 4, 6, 8} : std::vector<uint32_t>{8, 12, 16, 24, 32};
+    const std::vector<uint32_t> spacings = cover == 128 ? std::vector<uint32_t>{4, 6, 8, 12} : std::vector<uint32_t>{8, 12, 16, 24, 32};
     for (int threads : {512, 256}) {
         for (uint32_t g : spacings) {
             const uint32_t most = cover == 128 ? (batch ? defaultTileSplitters<128, true>(g, threads) : defaultTileSplitters<128, false>(g, threads))

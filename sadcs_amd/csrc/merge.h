@@ -35,11 +35,10 @@ constexpr int kMaxLists = 32;
 // inside a tile reads the text; with mod 128 the rest comes from global memory.
 constexpr int kRecordWords = 4;
 
-// The tie-offset and window-slot tables a cover-128 comparison reads, flattened [a * V + b], in
-// __constant__ memory (cover 32 computes both by bit arithmetic).
+// The tie-offset table a cover-128 comparison reads, flattened [a * V + b], in __constant__
+// memory (cover 32 computes tie offsets by bit arithmetic).
 struct TieTables {
     const uint8_t* tie;
-    const uint8_t* slot;
 };
 
 // Cover 32 as a bit mask: bit k of rotr(mask, r) says whether r + k is a sample position (mod 32).
@@ -138,7 +137,7 @@ struct ListSet {
 template <int V, bool kBatch>
 struct MergeRecord {
     uint64_t words[kRecordWords];
-    uint32_t ranks[CoverSet<V>::kSize];
+    uint32_t ranks[CoverSet<V>::kSize];  // the rank window, where recordRanks<V>()
     uint32_t position;
     uint32_t tag;  // batch: segment << 9 | min(remaining, V)
 };
@@ -146,21 +145,31 @@ struct MergeRecord {
 template <int V>
 struct MergeRecord<V, false> {
     uint64_t words[kRecordWords];
-    uint32_t ranks[CoverSet<V>::kSize];
+    uint32_t ranks[CoverSet<V>::kSize];  // the rank window, where recordRanks<V>()
     uint32_t position;
 };
 
+// Whether a record holds its rank window. Cover 32 does: its window (7 ranks) lies in the record's
+// line. Cover 128 does not: 13 ranks would more than double a record's LDS (88 bytes instead of
+// 36) and halve the records a tile holds, while its comparisons rarely reach the ranks (only when
+// the first k characters tie, k averaging 59); those that do read the two ranks from the rank
+// array.
+template <int V>
+constexpr bool recordRanks() {
+    return V == 32;
+}
+
 // A record with few, wide gathers, in two steps so that a caller can issue the loads early and use
 // them late: fetchRecord issues three 16-byte loads holding T[p - 1 .. p + 31] and kQuads 16-byte
-// loads holding the rank window; decodeRecord shifts them into place. From the lines (cover 32)
-// all of them lie in one 128-byte line; otherwise they come from the text and the rank array
-// (16-byte aligned and padded past their ends). `below` is the number of cover elements below
-// p mod V (read by the caller from whichever copy of the table is closest).
+// loads holding the rank window (none without one); decodeRecord shifts them into place. From the
+// lines (cover 32) all of them lie in one 128-byte line; otherwise they come from the text and the
+// rank array (16-byte aligned and padded past their ends). `below` is the number of cover elements
+// below p mod V (read by the caller from whichever copy of the table is closest).
 template <int V>
 struct RawRecord {
-    static constexpr int kQuads = (CoverSet<V>::kSize + 3 + 3) / 4;  // 16-byte loads covering kSize words from any offset
+    static constexpr int kQuads = recordRanks<V>() ? (CoverSet<V>::kSize + 3 + 3) / 4 : 0;  // 16-byte loads covering kSize words from any offset
     uint4 chunk[3];
-    uint4 quad[kQuads];
+    uint4 quad[kQuads > 0 ? kQuads : 1];
 };
 
 template <int V>
@@ -235,17 +244,19 @@ __device__ __forceinline__ void decodeRecord(const MergeCtx& c, const Texts& tex
     for (int k = 0; k < kRecordWords; k++) {
         r.words[k] = __builtin_bswap64(s ? (v[k] >> s) | (v[k + 1] << (64 - s)) : v[k]);
     }
-    uint32_t x[4 * kQuads];
+    if constexpr (kQuads > 0) {
+        uint32_t x[4 * kQuads];
 #pragma unroll
-    for (int q = 0; q < kQuads; q++) {
-        x[4 * q] = raw.quad[q].x;
-        x[4 * q + 1] = raw.quad[q].y;
-        x[4 * q + 2] = raw.quad[q].z;
-        x[4 * q + 3] = raw.quad[q].w;
-    }
+        for (int q = 0; q < kQuads; q++) {
+            x[4 * q] = raw.quad[q].x;
+            x[4 * q + 1] = raw.quad[q].y;
+            x[4 * q + 2] = raw.quad[q].z;
+            x[4 * q + 3] = raw.quad[q].w;
+        }
 #pragma unroll
-    for (int s2 = 0; s2 < kSize; s2++) {
-        r.ranks[s2] = o == 0 ? x[s2] : o == 1 ? x[s2 + 1] : o == 2 ? x[s2 + 2] : x[s2 + 3];
+        for (int s2 = 0; s2 < kSize; s2++) {
+            r.ranks[s2] = o == 0 ? x[s2] : o == 1 ? x[s2 + 1] : o == 2 ? x[s2 + 2] : x[s2 + 3];
+        }
     }
     r.position = p;
     if constexpr (Texts::kBatch) {
@@ -306,14 +317,18 @@ __device__ __forceinline__ bool lessThanRecord(const MergeCtx& c, const Texts& t
     if (common < k || ra == k || rb == k) {
         return i > j;
     }
-    // The record lives in registers: select its rank without indexing it dynamically.
-    const uint32_t slot = cv.slot[j & (V - 1)][k];
-    uint32_t rankB = 0;
+    if constexpr (!recordRanks<V>()) {
+        return c.ranks[packPosition<V>(i + k)] < c.ranks[packPosition<V>(j + k)];
+    } else {
+        // The record lives in registers: select its rank without indexing it dynamically.
+        const uint32_t slot = cv.slot[j & (V - 1)][k];
+        uint32_t rankB = 0;
 #pragma unroll
-    for (int s = 0; s < CoverSet<V>::kSize; s++) {
-        rankB = s == static_cast<int>(slot) ? b.ranks[s] : rankB;
+        for (int s = 0; s < CoverSet<V>::kSize; s++) {
+            rankB = s == static_cast<int>(slot) ? b.ranks[s] : rankB;
+        }
+        return c.ranks[packPosition<V>(i + k)] < rankB;
     }
-    return c.ranks[packPosition<V>(i + k)] < rankB;
 }
 
 // The list holding concatenated index `index`: the number of list starts after the first at or
@@ -608,12 +623,11 @@ constexpr int kMergeThreads = 512;
 
 // The defaults per cover, measured on one MI355X (16 MiB to 1 GiB texts). A tile holds up to
 // K*G + C*(G-1) records for C lists, C*(G-1) of them the room its unaligned run ends need, so the
-// 31 lists of cover 128 want a small G: at G = 16 that room is 465 of a 512-thread tile's 865
-// records. Cover 128 merges in 256-thread tiles at G = 4 (four tiles per CU, 429 records each),
-// 1.9 times faster than in 512-thread tiles at G = 16; cover 32 (7 lists) in 512-thread tiles
-// at G = 16.
+// 31 lists of cover 128 want a smaller G than the 7 of cover 32: at G = 16 that room would be 465
+// of the 973 records of a 256-thread tile. Cover 128 merges in 256-thread tiles (four per CU) at
+// G = 8; cover 32 in 512-thread tiles (two per CU) at G = 16.
 constexpr uint32_t defaultSplitterSpacing(int cover) {
-    return cover == 128 ? 4 : 16;
+    return cover == 128 ? 8 : 16;
 }
 constexpr int defaultTileThreads(int cover) {
     return cover == 128 ? 256 : kMergeThreads;
@@ -634,8 +648,9 @@ struct LdsRecords {
     uint32_t* position;  // [capacity], or (position, tag) pairs [capacity][2] for a batch
     uint32_t* ranks;     // [kSize][capacity]
 
+    static constexpr int kStoredRanks = recordRanks<V>() ? kSize : 0;
     static constexpr size_t bytesPerRecord() {
-        return 8 * kRecordWords + 4 + 4 * kSize + (kBatch ? 4 : 0);
+        return 8 * kRecordWords + 4 + 4 * kStoredRanks + (kBatch ? 4 : 0);
     }
 
     __device__ __forceinline__ void carve(unsigned char* base, uint32_t cap) {
@@ -669,7 +684,7 @@ struct LdsRecords {
             position[e] = r.position;
         }
 #pragma unroll
-        for (int s = 0; s < kSize; s++) {
+        for (int s = 0; s < kStoredRanks; s++) {
             ranks[s * capacity + e] = r.ranks[s];
         }
     }
@@ -810,7 +825,8 @@ __device__ __forceinline__ bool lessLds(const MergeCtx& c, const TieTables& t, c
         if (common < k || ra == k || rb == k) {
             return i > j;  // one is a prefix of the other: the shorter suffix is smaller
         }
-        return R.ranks[t.slot[(i & (V - 1)) * V + k] * cap + x] < R.ranks[t.slot[(j & (V - 1)) * V + k] * cap + y];
+        // Both suffixes run past i + k and j + k, sample positions: their ranks decide.
+        return c.ranks[packPosition<V>(i + k)] < c.ranks[packPosition<V>(j + k)];
     }
 }
 
@@ -910,19 +926,19 @@ __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const TieTables& t
     }
 }
 
-// The comparison tables of a tile merge for covers other than 32 (cover 32 computes them).
+// The comparison table of a tile merge for covers other than 32 (cover 32 computes it).
 template <int V>
 __device__ __forceinline__ TieTables setupTables() {
-    const CoverTables<V>& cv = deviceCover<V>();
-    return TieTables{&cv.tie[0][0], &cv.slot[0][0]};
+    return TieTables{&deviceCover<V>().tie[0][0]};
 }
 
 // Merges the runs of indices in src (run starts bounds[0..runs), bounds[runs] = size) in LDS by
 // rounds of pairwise merges; returns the buffer holding the merged order. afterRound() runs after
 // every round, between barriers, on every thread. identityFirst: src[i] = i, so the first round
 // need not read it. Each thread takes a chunk of at least two consecutive outputs: a tile with
-// fewer records than 2 * kThreads leaves the rest of its threads idle, and at cover 128 (tiles of
-// about 340 records in 256 threads) chunks of 2 merged 5% faster than chunks of 3.
+// fewer records than 2 * kThreads leaves the rest of its threads idle, and in tiles of about 340
+// records in 256 threads (cover 128 with G = 4 and rank windows in the records) chunks of 2
+// merged 5% faster than chunks of 3.
 template <int V, class Texts, int kThreads, class AfterRound>
 __device__ __forceinline__ uint16_t* mergeRuns(const MergeCtx& c, const TieTables& tables, const LdsRecords<V, Texts::kBatch>& recs, uint16_t* src, uint16_t* dst, uint32_t* bounds, uint32_t runs, uint32_t size, bool identityFirst, AfterRound afterRound) {
     const uint32_t chunk = max(2u, (size + kThreads - 1) / kThreads);

@@ -225,7 +225,7 @@ def test_merge_parameters(gpu):
 
 
 def test_merge_parameters_cover_128(gpu):
-    """Cover 128 (31 lists) with its default tiles (G = 4, 256 threads), the cover-32 defaults
+    """Cover 128 (31 lists) with its default tiles (G = 8, 256 threads), the cover-32 defaults
     (G = 16, 512 threads) and small tiles that force several merge levels."""
     rng = oracle.Rng(19)
     text = rng.random_text(400_000, 4)
