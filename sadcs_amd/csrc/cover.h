@@ -139,6 +139,7 @@ struct CoverTables {
     uint16_t below[V];         // samples in [0, r)
     uint16_t residue[kSize];   // the cover, ascending
     uint8_t distance[V];       // class of residue r
+    uint8_t anchor[V];         // [d]: the smallest cover element x with x + d (mod V) in the cover
 
     void fill() {
         using C = DifferenceCover<V>;
@@ -154,6 +155,14 @@ struct CoverTables {
         }
         for (int e = 0; e < kSize; e++) {
             residue[e] = static_cast<uint16_t>(CoverSet<V>::kElements[e]);
+        }
+        for (int d = 0; d < V; d++) {
+            anchor[d] = 0;
+            for (int e = kSize - 1; e >= 0; e--) {
+                if (C::isSample(CoverSet<V>::kElements[e] + d)) {
+                    anchor[d] = static_cast<uint8_t>(CoverSet<V>::kElements[e]);
+                }
+            }
         }
     }
 };

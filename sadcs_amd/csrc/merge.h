@@ -35,10 +35,12 @@ constexpr int kMaxLists = 32;
 // inside a tile reads the text; with mod 128 the rest comes from global memory.
 constexpr int kRecordWords = 4;
 
-// The tie-offset table a cover-128 comparison reads, flattened [a * V + b], in __constant__
-// memory (cover 32 computes tie offsets by bit arithmetic).
+// The tables a tile comparison reads for covers other than 32 (cover 32 computes tie offsets by
+// bit arithmetic): the tie offsets, flattened [a * V + b], in __constant__ memory, and per
+// difference d an anchor, a cover element x with x + d in the cover (cover 128: a copy in LDS).
 struct TieTables {
     const uint8_t* tie;
+    const uint8_t* anchor;
 };
 
 // Cover 32 as a bit mask: bit k of rotr(mask, r) says whether r + k is a sample position (mod 32).
@@ -806,7 +808,15 @@ __device__ __forceinline__ bool lessLds(const MergeCtx& c, const TieTables& t, c
             ra = min(c.n - i, static_cast<uint32_t>(V));
             rb = min(c.n - j, static_cast<uint32_t>(V));
         }
-        const uint32_t k = t.tie[(i & (V - 1)) * V + (j & (V - 1))];
+        uint32_t k;
+        if constexpr (V == 128) {
+            // i + k = x and j + k = x + d (mod 128) are both samples, for the anchor x of the
+            // difference d = j - i: one LDS byte instead of the tie table in global memory. Not
+            // always the smallest such k, but any k with both sampled orders the two the same.
+            k = (t.anchor[(j - i) & (V - 1)] - i) & (V - 1);
+        } else {
+            k = t.tie[(i & (V - 1)) * V + (j & (V - 1))];
+        }
         const uint32_t common = min(k, min(ra, rb));
         const uint32_t cap = R.capacity;
 
@@ -926,10 +936,10 @@ __device__ __forceinline__ void mergeChunk(const MergeCtx& c, const TieTables& t
     }
 }
 
-// The comparison table of a tile merge for covers other than 32 (cover 32 computes it).
+// The comparison tables of a tile merge for covers other than 32 (cover 32 computes them).
 template <int V>
 __device__ __forceinline__ TieTables setupTables() {
-    return TieTables{&deviceCover<V>().tie[0][0]};
+    return TieTables{&deviceCover<V>().tie[0][0], &deviceCover<V>().anchor[0]};
 }
 
 // Merges the runs of indices in src (run starts bounds[0..runs), bounds[runs] = size) in LDS by
@@ -1141,7 +1151,14 @@ __global__ __launch_bounds__(kThreads) void mergeTilesKernel(MergeCtx c, Texts t
 
     const int lc = lists.count;
     const uint64_t tile = static_cast<uint64_t>(firstTile) + blockIdx.x;
-    const TieTables tables = setupTables<V>();
+    TieTables tables = setupTables<V>();
+    if constexpr (V == 128) {
+        __shared__ uint8_t anchorLds[V];
+        if (threadIdx.x < static_cast<uint32_t>(V)) {
+            anchorLds[threadIdx.x] = tables.anchor[threadIdx.x];
+        }
+        tables.anchor = anchorLds;  // read after the barrier below
+    }
 
     if (threadIdx.x < static_cast<uint32_t>(lc)) {
         listPtr[threadIdx.x] = lists.ptr[threadIdx.x];
