@@ -408,9 +408,9 @@ class ScratchStack {
 
 struct Params {
     int cover = 32;
-    uint32_t g = 16;        // splitter spacing
-    uint32_t k = 0;         // sorted splitters per tile; 0: the most that keep the CU's tiles in its LDS
-    int threads = kMergeThreads;  // threads per merge tile (512 or 256)
+    uint32_t g = defaultSplitterSpacing(32);  // splitter spacing
+    uint32_t k = 0;                           // sorted splitters per tile; 0: the most that keep the CU's tiles in its LDS
+    int threads = defaultTileThreads(32);     // threads per merge tile (512 or 256)
 
     // Zero fields take the defaults of the cover.
     void apply(const sadcs_params* p) {
