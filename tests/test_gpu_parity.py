@@ -284,6 +284,23 @@ def test_determinism(gpu):
     assert np.array_equal(a[0], b[0]) and a[1:] == b[1:]
 
 
+def test_repeated_fused_builds_cover_128(gpu):
+    """The fused SA + BWT merge at cover 128, built four times over texts with long repeats (many
+    comparisons past the records' 32 characters and down to the ranks): every build equals the
+    reference CPU build."""
+    from sadcs_amd import workloads
+    for name, text in [("dna-like", workloads.dna_like(3 << 20, 41).tobytes()),
+                       ("markov", workloads.markov(3 << 20, 42).tobytes())]:
+        want = expected_sa(text)
+        want_bwt = oracle.bwt_from_sa(text, want)
+        with gpu.GpuContext(len(text), cover=128) as ctx:
+            for r in range(4):
+                print(f"cover 128 fused {name} build {r}", flush=True)
+                sa, b, primary = gpu.gpu_build(text, True, True, context=ctx)
+                assert np.array_equal(sa, want), f"{name} build {r}: SA differs"
+                assert (b, primary) == want_bwt, f"{name} build {r}: BWT differs"
+
+
 def test_serving_results_equal_alone_and_in_mix(gpu):
     """Requests submitted together are batched and overlapped; each result must equal the same text
     built alone (and the oracle)."""
